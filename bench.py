@@ -1,0 +1,177 @@
+"""Benchmark of the fermion-matrix hot path on one MI355X.
+
+Workload (default): 4D 16^3 x 32 SU(3) Wilson D^dag D, kappa = 0.13, links
+U = exp(i 0.3 H) (seeded), one Gaussian source, multishift CG over 12 shifts
+log-spaced in [1e-4, 10], every shift to true relative residual 1e-10
+(recomputed in FP64).  One step = one complete multishift solve.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|double]
+
+Prints one JSON line (metric, value = solves/s, ms_per_step, p50/p99 solve
+latency, Wilson-kernel HBM GB/s, roofline of the dominant kernel, CPU
+baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    "c2": dict(geometry=(16, 16, 16, 32), nc=3, kappa=0.13, eps=0.3, nshift=12, lo=1e-4, hi=10.0,
+               name="wilson_ddag_d_16c3x32_su3_multishift12"),
+    "c5": dict(geometry=(32, 32, 32, 64), nc=3, kappa=0.1340, eps=0.2, nshift=24, lo=1e-5, hi=20.0,
+               name="wilson_ddag_d_32c3x64_su3_multishift24"),
+    "c3": dict(geometry=(24, 24, 24, 48), nc=3, kappa=0.1335, eps=0.25, nshift=12, lo=1e-4, hi=10.0,
+               name="wilson_ddag_d_24c3x48_su3_multishift12"),
+}
+
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_site_dslash(nc, ndim, wl, ws):
+    """Compulsory bytes of one D application per site: links read once,
+    input and output spinor once (w = bytes per real)."""
+    ns = 4 if ndim == 4 else 2
+    return ndim * nc * nc * 2 * wl + 2 * ns * nc * 2 * ws
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "double"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--dslash-reps", type=int, default=50)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU domain decomposition "
+              "is not built in this version", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from tfleonardym_amd import _lib, dirac, dirac_solver, fields, lattice
+    import ctypes
+
+    dev = _lib.require_device()
+    W = WORKLOADS[args.workload]
+    geometry, nc = W["geometry"], W["nc"]
+    V = int(np.prod(geometry))
+    t0 = time.time()
+    U = fields.gauge_field(geometry, nc, seed=20240, kind="near_identity", eps=W["eps"])
+    b = fields.gaussian_spinor(geometry, 4, nc, seed=20241)
+    shifts = fields.log_shifts(W["lo"], W["hi"], W["nshift"])
+    print(f"[bench] inputs generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    cfg = lattice.Configuration(U, geometry, nc, 4)
+    op = dirac.DiracWilsonOperator(cfg, W["kappa"], True, representation="fundamental")
+    A = dirac.SquareOperator(op)
+    bd = torch.from_numpy(b).to(dev)
+    solver = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10,
+                                            precision_mode=args.precision, verbose=False)
+
+    # ---- warmup + timed solves ------------------------------------------------
+    for i in range(args.warmup):
+        solver.solve(A, bd, shifts)
+        print(f"[bench] warmup {i} done: {solver.last_stats['iterations']} it, "
+              f"{solver.last_stats['seconds']:.3f}s", file=sys.stderr, flush=True)
+    lat = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    stats = None
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        xs = solver.solve(A, bd, shifts)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+        stats = solver.last_stats
+        if stats["status"] != "converged" or max(stats["true_rel_resid"]) > 1e-10:
+            print(f"[bench] solve {i} did not reach 1e-10: {stats}", file=sys.stderr)
+            sys.exit(1)
+        print(f"[bench] step {i}: {lat[-1] * 1e3:.1f} ms, {stats['iterations']} it, restarts {stats['restarts']}",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t_start
+    solves_per_s = args.steps / total
+
+    # ---- Wilson kernel bandwidth (HIP events on the launch stream) ----------
+    desc = op.ensure_f32()
+    res = {}
+    stream = torch.cuda.current_stream()
+    for prec, name, wl in ((_lib.PREC_F64, "f64", 8), (_lib.PREC_F32, "f32", 4)):
+        dt = torch.complex128 if prec == _lib.PREC_F64 else torch.complex64
+        x = bd.to(dt)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), prec, 0, _lib.ptr(x), _lib.ptr(y),
+                                                   _lib.stream_handle()))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.dslash_reps):
+            _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), prec, 0, _lib.ptr(x), _lib.ptr(y),
+                                                   _lib.stream_handle()))
+        e1.record(stream)
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / args.dslash_reps
+        bps = bytes_per_site_dslash(nc, 4, wl, wl)
+        res[name] = dict(us=us, bytes_per_site=bps, gbs=bps * V / (us * 1e-6) / 1e9)
+        print(f"[bench] wilson {name}: {us:.2f} us/launch, {res[name]['gbs']:.0f} GB/s", file=sys.stderr,
+              flush=True)
+
+    # ---- CPU baseline: the oracle (NumPy restatement of the reference op
+    # sequence), a bounded number of FP64 CG-M iterations, extrapolated ------
+    cpu = None
+    if not args.no_cpu:
+        from oracle import solvers as osv
+        from oracle import wilson as ow
+        oop = ow.WilsonOracle(U, geometry, W["kappa"], True, "fundamental")
+        tc = time.perf_counter()
+        osv.cgm(oop.apply_square, b, shifts, tol=1e-10, max_iter=args.cpu_iters)
+        per_it = (time.perf_counter() - tc) / args.cpu_iters
+        its = stats["iterations"]
+        cpu = {"value": 1.0 / (per_it * its), "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_iters} CG-M iterations (12 shifts) of this solve by the NumPy complex128 "
+                         f"oracle following the reference op sequence, {per_it:.2f} s/iteration, extrapolated "
+                         f"to the {its} iterations of the GPU solve"}
+        print(f"[bench] cpu baseline {per_it:.2f} s/iteration", file=sys.stderr, flush=True)
+
+    f32 = res["f32"]
+    out = {
+        "metric": "multishift-CG solves/s at 1e-10 true resid",
+        "value": solves_per_s,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded near-identity SU(3) links, Gaussian source)",
+        "config": {"workload": W["name"], "geometry": list(geometry), "kappa": W["kappa"], "shifts": len(shifts),
+                   "shift_range": [W["lo"], W["hi"]], "precision": args.precision, "tolerance": 1e-10},
+        "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
+        "iterations": stats["iterations"],
+        "max_true_rel_resid": max(stats["true_rel_resid"]),
+        "dslash_gbs": {"f64": res["f64"]["gbs"], "f32": f32["gbs"]},
+        "roofline": {"kernel": "k_wilson f32 (D, one RHS)", "bound": "hbm", "achieved": f32["gbs"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": f32["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_site": f32["bytes_per_site"], "sites": V, "avg_us": f32["us"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,152 @@
+/*
+ * lym.h -- C ABI of liblym.so, the MI355X (gfx950) implementation of the
+ * TfLeonardYM fermion-matrix hot path: the Wilson-Dirac operator D / Q = g5 D,
+ * D^dag D, the reference's multishift solver, and (new) CG / multishift CG.
+ *
+ * Conventions shared by every entry point
+ *   - All field pointers are DEVICE pointers.  The library borrows them for
+ *     the duration of the call and keeps nothing.
+ *   - Layouts are the reference's (site index last, last lattice coordinate
+ *     fastest, lattice.py:92-114):
+ *       links               (ndim, nc, nc, V) complex
+ *       fundamental spinor  (Ns, nc, V) complex          Ns = 2 (2D) or 4 (4D)
+ *       adjoint spinor      (Ns, nc, nc, V) complex matrices
+ *   - Precision LYM_PREC_F64 = complex128, LYM_PREC_F32 = complex64.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Calls are
+ *     asynchronous on that stream unless stated otherwise.
+ *   - Return value: 0 OK, <0 error (see lym_last_error()), >0 solver status
+ *     (not converged / breakdown).
+ */
+#ifndef LYM_H
+#define LYM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LYM_ABI_VERSION 1
+
+enum {
+  LYM_OK = 0,
+  LYM_ERR_ARG = -1,
+  LYM_ERR_HIP = -2,
+  LYM_ERR_NOMEM = -3,
+  LYM_ERR_UNSUPPORTED = -4,
+  LYM_NOT_CONVERGED = 1,
+  LYM_BREAKDOWN = 2
+};
+
+enum { LYM_PREC_F64 = 0, LYM_PREC_F32 = 1, LYM_PREC_F16 = 2 };
+enum { LYM_REP_FUNDAMENTAL = 0, LYM_REP_ADJOINT = 1 };
+enum { LYM_MODE_PROJECTED = 0, LYM_MODE_SIMPLE = 1 };
+
+/* Operator description (replaces the state DiracWilsonOperator builds in
+ * dirac.py:254-378).  `links` are the gauge links as stored by the caller;
+ * the antiperiodic boundary factor (boundary_conditions.py:23-46) is applied
+ * on the fly to U_{ndim-1}(x) at x_{ndim-1} = L-1 when (bc_re, bc_im) != (1,0).
+ * Adjoint representation: the hop is U_left X U_right^dag (dirac.py:416-426);
+ * links_right = NULL means U_right = links without the boundary factor. */
+typedef struct lym_wilson {
+  int32_t ndim;           /* 2 or 4 (dirac.py:299-300) */
+  int32_t nc;             /* 2 or 3 */
+  int32_t representation; /* LYM_REP_* */
+  int32_t hermitian;      /* 1: Q = g5 D (dirac.py:649-662) */
+  int32_t dims[4];        /* lattice extents, direction order (--geometry) */
+  double kappa;           /* D = 1 + kappa * hopping (dirac.py:458-523) */
+  double bc_re, bc_im;    /* boundary factor, (1,0) = periodic */
+  const void* links;            /* complex128 */
+  const void* links_right;      /* complex128 or NULL (adjoint only) */
+  const void* links_f32;        /* complex64 copy of links, or NULL */
+  const void* links_right_f32;  /* complex64 copy of links_right, or NULL */
+} lym_wilson;
+
+typedef struct lym_solve_stats {
+  int32_t status;      /* 1 converged, 2 max iterations, 3 breakdown */
+  int32_t iterations;  /* iterations of the (base) recurrence */
+  int32_t nshift;
+  int32_t restarts;    /* reliable updates / refinement cycles */
+  double final_rr;     /* last iterated <r,r> of the base system */
+  double seconds;      /* wall time of the call (host clock, synchronised) */
+  int32_t conv_iter[64];
+  double true_rel_resid[64]; /* ||b - (A+s)x|| / ||b||, FP64, when computed */
+} lym_solve_stats;
+
+/* ---- runtime --------------------------------------------------------- */
+int lym_abi_version(void);
+/* Select `device` for this host thread's subsequent calls; no-op if current. */
+int lym_init(int device);
+/* Message of the last error on this host thread ("" if none). */
+const char* lym_last_error(void);
+
+/* ---- operator ---------------------------------------------------------
+ * out = D in, or Q in when op->hermitian.
+ * replaces DiracWilsonOperator.apply (dirac.py:270-277), i.e.
+ * multiply_in_fundamental_representation (dirac.py:458-501),
+ * simple_multiply_in_fundamental_representation (dirac.py:503-523),
+ * multiply_in_adjoint_representation (dirac.py:404-456),
+ * simple_multiply_in_adjoint_representation (dirac.py:380-402).
+ * mode: LYM_MODE_PROJECTED (half spinors) or LYM_MODE_SIMPLE (full
+ * projectors; F64 only).  `in` and `out` must not overlap. */
+int lym_wilson_apply(const lym_wilson* op, int prec, int mode, const void* in, void* out, void* stream);
+
+/* out = Op(Op(in)) + shift * in, using `tmp` (same shape) as scratch.
+ * replaces SquareOperator.apply (dirac.py:36-37) composed with
+ * ShiftedOperator.apply (dirac.py:24-25). */
+int lym_wilson_apply_square(const lym_wilson* op, int prec, double shift, const void* in, void* out,
+                            void* tmp, void* stream);
+
+/* ---- BLAS-1 over n complex numbers ------------------------------------
+ * lym_dot: result[0] + i result[1] = sum conj(x) y, accumulated in FP64,
+ * deterministic; synchronises `stream` and writes `result` on the host.
+ * replaces algebra_utils.dot (algebra_utils.py:5-11) + mth_global_sum
+ * (lattice.py:375-376). `scratch` >= lym_reduce_scratch_bytes(n) bytes. */
+size_t lym_reduce_scratch_bytes(int64_t n);
+int lym_dot(int prec, int64_t n, const void* x, const void* y, double* result, void* scratch, void* stream);
+/* y = a x + b y (complex scalars); replaces the solver updates written as
+ * tensor arithmetic in dirac_solver.py:33-45 and dirac_solver.py:83-97. */
+int lym_axpby(int prec, int64_t n, double a_re, double a_im, const void* x, double b_re, double b_im, void* y,
+              void* stream);
+/* Negate components with spin >= Ns/2 of `n_per_spin * ns` complex numbers;
+ * replaces apply_hermiticity / apply_gammad (dirac.py:649-662, 686-696). */
+int lym_gamma5(int prec, int ns, int64_t n_per_spin, const void* in, void* out, void* stream);
+/* Precision conversion of n complex numbers (F64 <-> F32). No reference
+ * counterpart: the reference runs in complex128 only. */
+int lym_convert(int prec_in, const void* in, int prec_out, void* out, int64_t n, void* stream);
+
+/* ---- solvers ---------------------------------------------------------- */
+/* Device workspace (bytes) for the solvers below. */
+size_t lym_solver_workspace_bytes(const lym_wilson* op, int prec, int nshift);
+
+/* Multishift minimal-residual solver exactly as multishift_solver.solve
+ * (dirac_solver.py:13-53) for A = Op^2 (SquareOperator): x_i = (A + s_i)^-1 b
+ * for the non-negative shifts; base shift = shifts[nshift-1]; stops when
+ * |<r,r>| < precision (absolute, iterated).  Leading negative shifts are
+ * NOT handled here (the host binding routes them to BiCGStab as
+ * dirac_solver.py:26-30 does); `first` is the index of the first shift
+ * solved here.  F64 only.  Synchronous. */
+int lym_mr_multishift(const lym_wilson* op, int nshift, const double* shifts, int first, const void* b,
+                      void* const* xs, int max_steps, double precision, double omega, void* workspace,
+                      size_t workspace_bytes, lym_solve_stats* stats, void* stream);
+
+/* Multishift conjugate gradient (Jegerlehner, hep-lat/9612014) on
+ * A = Q^2 + s_i (requires op->hermitian): x_i = (Q^2 + s_i)^-1 b, all shifts
+ * >= 0, any order (the base is the smallest).  Stops each shift when its
+ * iterated relative residual <= tol; then recomputes the FP64 true relative
+ * residual of every shift into stats->true_rel_resid.
+ *   prec = LYM_PREC_F64: FP64 throughout.
+ *   prec = LYM_PREC_F32: FP32 CG-M to tol_inner, then FP64 defect
+ *          correction of every shift (FP32 inner CG) to tol.
+ * b and xs are complex128.  Synchronous.  New entry point (no counterpart in
+ * the reference, whose multishift solver is the MR above); it serves
+ * multishift_solver.solve / RationalApproximation.apply (dirac.py:52-59). */
+int lym_cgm(const lym_wilson* op, int prec, int nshift, const double* shifts, const void* b, void* const* xs,
+            double tol, int max_iter, void* workspace, size_t workspace_bytes, lym_solve_stats* stats,
+            void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LYM_H */

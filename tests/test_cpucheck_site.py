@@ -1,0 +1,77 @@
+"""CPU check of the kernels' per-site Wilson arithmetic (wilson_site.h compiled
+for the host, looped over the lattice) against the oracle: site indexing,
+gamma tables, colour transport and the in-kernel boundary factor, for every
+(ndim, Nc, representation, hermitian, boundary, mode) the kernels build."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from oracle import wilson as ow
+from tfleonardym_amd import fields
+
+LIB = os.path.join(os.path.dirname(__file__), "..", "tfleonardym_amd", "liblym_cpucheck.so")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        from tfleonardym_amd import build
+        build.build_cpu_check()
+    lib = ctypes.CDLL(LIB)
+    lib.lymcheck_wilson_apply.restype = ctypes.c_int
+    return lib
+
+
+def run(lib, geometry, nc, adj, herm, kappa, bc, proj, links, links_r, psi):
+    ndim = len(geometry)
+    dims = (ctypes.c_int * 4)(*(list(geometry) + [1] * (4 - ndim)))
+    out = np.zeros_like(psi)
+    links = np.ascontiguousarray(links)
+    lr = None if links_r is None else np.ascontiguousarray(links_r)
+    rc = lib.lymcheck_wilson_apply(ndim, nc, adj, herm, dims, ctypes.c_double(kappa), ctypes.c_double(bc.real),
+                                   ctypes.c_double(bc.imag), proj, links.ctypes.data_as(ctypes.c_void_p),
+                                   None if lr is None else lr.ctypes.data_as(ctypes.c_void_p),
+                                   np.ascontiguousarray(psi).ctypes.data_as(ctypes.c_void_p),
+                                   out.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    return out
+
+
+CASES = []
+for geometry in [(4, 6), (3, 4), (2, 4, 2, 4), (4, 2, 3, 4)]:
+    for nc in (2, 3):
+        for adj in (0, 1):
+            CASES.append((geometry, nc, adj))
+
+
+@pytest.mark.parametrize("geometry,nc,adj", CASES)
+@pytest.mark.parametrize("herm", [0, 1])
+@pytest.mark.parametrize("bc", [None, -1.0, np.exp(0.3j)])
+@pytest.mark.parametrize("proj", [1, 0])
+def test_site_arithmetic_matches_oracle(lib, geometry, nc, adj, herm, bc, proj):
+    D = len(geometry)
+    ns = 2 if D == 2 else 4
+    rep = "adjoint" if adj else "fundamental"
+    U = fields.gauge_field(geometry, nc, seed=13, kind="haar")
+    psi = fields.gaussian_spinor(geometry, ns, nc, seed=4, representation=rep)
+    kappa = 0.1234
+    op = ow.WilsonOracle(U, geometry, kappa, hermitian=bool(herm), representation=rep, bc_factor=bc)
+    want = op.apply(psi)
+    got = run(lib, geometry, nc, adj, herm, kappa, complex(1.0 if bc is None else bc), proj, U, None, psi)
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert err < 1e-14, err
+
+
+def test_adjoint_custom_provider_two_link_fields(lib):
+    """links_right != links: left field carries an arbitrary provider's change."""
+    geometry = (4, 4)
+    nc = 2
+    U = fields.gauge_field(geometry, nc, seed=3, kind="haar")
+    X = fields.gaussian_spinor(geometry, 2, nc, seed=1, representation="adjoint")
+    Ul = U.copy()
+    Ul[0] *= np.exp(0.7j)  # a provider that twists direction 0 everywhere
+    want = ow.apply_projected_adjoint(Ul, U, X, 0.11, True, __import__("oracle").lattice.lookup_tables(geometry))
+    got = run(lib, geometry, nc, 1, 1, 0.11, 1.0 + 0j, 1, Ul, U, X)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-14

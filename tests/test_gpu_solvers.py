@@ -1,0 +1,148 @@
+"""GPU solvers against the oracle on identical seeded inputs.
+
+CG / CG-M: every shift reaches true relative residual <= 1e-10, recomputed
+here in FP64 by the oracle's operator on the returned solution, and agrees
+with the oracle's own solve.  Multishift MR: the reference recurrence; same
+iteration count as the oracle restatement (+-1) and solutions equal to
+1e-8 relative.  BiCGStab: residual and agreement with MR (tests.py:108-113).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+from oracle import solvers as osv
+from oracle import wilson as ow
+from tfleonardym_amd import fields
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else a
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def setup(geometry, nc, kappa, kind="near_identity", eps=0.3, bc=-1.0, rep="fundamental", seed=77):
+    from tfleonardym_amd import boundary_conditions, dirac, lattice
+    U = fields.gauge_field(geometry, nc, seed=seed, kind=kind, eps=eps)
+    cfg = lattice.Configuration(U, geometry, nc, len(geometry))
+    prov = None
+    if bc is not None:
+        boundary_conditions.set_antiperiodic_field_in_t_direction(geometry, factor=bc)
+        prov = boundary_conditions.apply_antiboundary_conditions_in_t_direction
+    op = dirac.DiracWilsonOperator(cfg, kappa, True, representation=rep, boundary_condition_provider=prov)
+    oop = ow.WilsonOracle(U, geometry, kappa, True, rep, bc_factor=bc)
+    return op, oop
+
+
+@pytest.mark.parametrize("mode", ["double", "mixed"])
+@pytest.mark.parametrize("geometry,nc", [((4, 4, 4, 4), 3), ((8, 8), 2), ((4, 4, 4, 8), 2)])
+def test_cgm_true_residual(mode, geometry, nc):
+    from tfleonardym_amd import dirac, dirac_solver
+    print(f"case cgm mode={mode} geometry={geometry} nc={nc}", flush=True)
+    op, oop = setup(geometry, nc, 0.13)
+    ns = 2 if len(geometry) == 2 else 4
+    b = fields.gaussian_spinor(geometry, ns, nc, seed=3)
+    shifts = fields.log_shifts(1e-3, 5.0, 6)
+    solver = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode=mode,
+                                            verbose=False)
+    xs = solver.solve(dirac.SquareOperator(op), b, shifts)
+    assert solver.last_status == "converged", solver.last_stats
+    ref, _ = osv.cgm(oop.apply_square, b, shifts, tol=1e-12)
+    for s, x, xr in zip(shifts, xs, ref):
+        xn = x.cpu().numpy()
+        assert osv.true_residual(oop.apply_square, b, xn, s) <= 1e-10, s
+        assert rel(xn, xr) < 1e-8
+    assert max(solver.last_stats["true_rel_resid"]) <= 1e-10
+
+
+def test_c1_plain_fp64_cg():
+    """The CPU-runnable workload: 2D 24x16 SU(2), Haar links, kappa=-0.15,
+    periodic x / antiperiodic t, FP64 CG on D^dag D to 1e-10."""
+    from tfleonardym_amd import dirac, dirac_solver
+    op, oop = setup((24, 16), 2, -0.15, kind="haar", seed=1)
+    b = fields.gaussian_spinor((24, 16), 2, 2, seed=2)
+    cg = dirac_solver.conjugate_gradient(10000, tolerance=1e-10)
+    x = cg.solve(dirac.SquareOperator(op), b).cpu().numpy()
+    assert osv.true_residual(oop.apply_square, b, x, 0.0) <= 1e-10
+    xr, it = osv.cg(oop.apply_square, b, 0.0, tol=1e-12)
+    assert rel(x, xr) < 1e-8
+
+
+@pytest.mark.parametrize("geometry,nc,rep", [((8, 8), 2, "fundamental"), ((4, 4, 4, 4), 3, "fundamental"),
+                                             ((4, 4), 2, "adjoint"), ((2, 2, 2, 4), 3, "adjoint")])
+def test_mr_multishift_matches_oracle(geometry, nc, rep):
+    """The reference's multishift MR recurrence on the device (dirac_solver.py:13-53)."""
+    from tfleonardym_amd import dirac, dirac_solver
+    op, oop = setup(geometry, nc, 0.1, kind="haar", rep=rep)
+    ns = 2 if len(geometry) == 2 else 4
+    b = fields.gaussian_spinor(geometry, ns, nc, seed=4, representation=rep)
+    shifts = [0.5, 0.1, 0.01]
+    solver = dirac_solver.multishift_solver(1000, 1e-11, verbose=False)
+    xs = solver.solve(dirac.SquareOperator(op), b, shifts)
+    ref, step, ok = osv.mr_multishift(oop.apply_square, b, shifts, 1000, 1e-11)
+    assert ok and solver.last_status == "converged"
+    assert abs(solver.last_stats["iterations"] - (step + 1)) <= 1
+    for x, xr in zip(xs, ref):
+        assert rel(x, xr) < 1e-8
+
+
+def test_mr_negative_shift_and_bicgstab():
+    from tfleonardym_amd import dirac, dirac_solver
+    op, oop = setup((4, 4, 4, 4), 2, 0.1, kind="haar")
+    b = fields.gaussian_spinor((4, 4, 4, 4), 4, 2, seed=5)
+    shifts = [-60.0, 0.3, 0.05]
+    solver = dirac_solver.multishift_solver(2000, 1e-12, verbose=False)
+    xs = solver.solve(dirac.SquareOperator(op), b, shifts)
+    ref, _, _ = osv.mr_multishift(oop.apply_square, b, shifts, 2000, 1e-12)
+    for x, xr in zip(xs, ref):
+        assert rel(x, xr) < 1e-7
+    # standalone BiCGStab vs the oracle restatement (tests.py:108-113)
+    sh = dirac.ShiftedOperator(dirac.SquareOperator(op), 0.05)
+    bi = dirac_solver.biconjugate_gradient(1000, 1e-12, verbose=False)
+    x = bi.solve(sh, b).cpu().numpy()
+    xr, step, ok = osv.bicgstab(lambda v: oop.apply_square(v, 0.05), b, 1000, 1e-12)
+    assert ok and bi.last_status == "converged"
+    assert abs(bi.last_steps - step) <= 1
+    assert rel(x, xr) < 1e-7
+    assert rel(x, xs[-1].cpu().numpy()) < 1e-6
+
+
+def test_rational_approximation_apply():
+    """r(A) v = shift v + sum alpha_i (A + beta_i)^-1 v (dirac.py:52-59) with the
+    force_level1 table (approximations.py:103-106)."""
+    from tfleonardym_amd import approximations, dirac, dirac_solver
+    op, oop = setup((8, 8), 2, 0.12, kind="haar")
+    b = fields.gaussian_spinor((8, 8), 2, 2, seed=6)
+    alphas, betas = approximations.split(approximations.force_level1_rational_approximation_coefficients)
+    for algo in ("mr", "cg"):
+        solver = dirac_solver.multishift_solver(5000, 1e-20, algorithm=algo, verbose=False)
+        ra = dirac.RationalApproximation(dirac.SquareOperator(op), alphas, betas, 0, solver)
+        got = ra.apply(b).cpu().numpy()
+        ref, _ = osv.cgm(oop.apply_square, b, betas, tol=1e-13)
+        want = sum(a * x for a, x in zip(alphas, ref))
+        assert rel(got, want) < 1e-8, algo
+
+
+def test_solve_is_deterministic():
+    from tfleonardym_amd import dirac, dirac_solver
+    op, _ = setup((4, 4, 4, 8), 3, 0.13)
+    b = fields.gaussian_spinor((4, 4, 4, 8), 4, 3, seed=8)
+    s = dirac_solver.multishift_solver(5000, 0, algorithm="cg", verbose=False)
+    x1 = s.solve(dirac.SquareOperator(op), b, [0.5, 0.01])
+    x2 = s.solve(dirac.SquareOperator(op), b, [0.5, 0.01])
+    for a, c in zip(x1, x2):
+        assert torch.equal(a, c)
+
+
+def test_solver_status_on_max_steps():
+    from tfleonardym_amd import dirac, dirac_solver
+    op, _ = setup((4, 4, 4, 4), 3, 0.13)
+    b = fields.gaussian_spinor((4, 4, 4, 4), 4, 3, seed=8)
+    s = dirac_solver.multishift_solver(3, 0, algorithm="cg", verbose=False)
+    s.solve(dirac.SquareOperator(op), b, [0.01])
+    assert s.last_status == "max_iterations"
+    m = dirac_solver.multishift_solver(3, 1e-30, verbose=False)
+    m.solve(dirac.SquareOperator(op), b, [0.5, 0.01])
+    assert m.last_status == "max_iterations"
+    assert m.last_stats["iterations"] == 3

@@ -1,0 +1,153 @@
+"""ctypes binding of liblym.so (the C ABI declared in include/lym.h).
+
+There is no fallback: if the library is missing or fails to load, importing
+the operator modules raises, so nothing silently runs off the GPU path.
+"""
+import ctypes
+import os
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblym.so")
+
+PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
+REP_FUNDAMENTAL, REP_ADJOINT = 0, 1
+MODE_PROJECTED, MODE_SIMPLE = 0, 1
+
+OK = 0
+NOT_CONVERGED = 1
+BREAKDOWN = 2
+STATUS_NAMES = {0: "running", 1: "converged", 2: "max_iterations", 3: "breakdown"}
+
+
+class LymWilson(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int32),
+        ("nc", ctypes.c_int32),
+        ("representation", ctypes.c_int32),
+        ("hermitian", ctypes.c_int32),
+        ("dims", ctypes.c_int32 * 4),
+        ("kappa", ctypes.c_double),
+        ("bc_re", ctypes.c_double),
+        ("bc_im", ctypes.c_double),
+        ("links", ctypes.c_void_p),
+        ("links_right", ctypes.c_void_p),
+        ("links_f32", ctypes.c_void_p),
+        ("links_right_f32", ctypes.c_void_p),
+    ]
+
+
+class LymSolveStats(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int32),
+        ("iterations", ctypes.c_int32),
+        ("nshift", ctypes.c_int32),
+        ("restarts", ctypes.c_int32),
+        ("final_rr", ctypes.c_double),
+        ("seconds", ctypes.c_double),
+        ("conv_iter", ctypes.c_int32 * 64),
+        ("true_rel_resid", ctypes.c_double * 64),
+    ]
+
+    def as_dict(self, nshift=None):
+        n = self.nshift if nshift is None else nshift
+        return {
+            "status": STATUS_NAMES.get(self.status, str(self.status)),
+            "iterations": int(self.iterations),
+            "restarts": int(self.restarts),
+            "final_rr": float(self.final_rr),
+            "seconds": float(self.seconds),
+            "conv_iter": [int(self.conv_iter[i]) for i in range(n)],
+            "true_rel_resid": [float(self.true_rel_resid[i]) for i in range(n)],
+        }
+
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int
+_i64 = ctypes.c_int64
+_f64 = ctypes.c_double
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes); every symbol include/lym.h declares
+SIGNATURES = {
+    "lym_abi_version": (_i32, []),
+    "lym_init": (_i32, [_i32]),
+    "lym_last_error": (ctypes.c_char_p, []),
+    "lym_wilson_apply": (_i32, [ctypes.POINTER(LymWilson), _i32, _i32, _vp, _vp, _vp]),
+    "lym_wilson_apply_square": (_i32, [ctypes.POINTER(LymWilson), _i32, _f64, _vp, _vp, _vp, _vp]),
+    "lym_reduce_scratch_bytes": (_sz, [_i64]),
+    "lym_dot": (_i32, [_i32, _i64, _vp, _vp, ctypes.POINTER(_f64), _vp, _vp]),
+    "lym_axpby": (_i32, [_i32, _i64, _f64, _f64, _vp, _f64, _f64, _vp, _vp]),
+    "lym_gamma5": (_i32, [_i32, _i32, _i64, _vp, _vp, _vp]),
+    "lym_convert": (_i32, [_i32, _vp, _i32, _vp, _i64, _vp]),
+    "lym_solver_workspace_bytes": (_sz, [ctypes.POINTER(LymWilson), _i32, _i32]),
+    "lym_mr_multishift": (_i32, [ctypes.POINTER(LymWilson), _i32, ctypes.POINTER(_f64), _i32, _vp,
+                                 ctypes.POINTER(_vp), _i32, _f64, _f64, _vp, _sz,
+                                 ctypes.POINTER(LymSolveStats), _vp]),
+    "lym_cgm": (_i32, [ctypes.POINTER(LymWilson), _i32, _i32, ctypes.POINTER(_f64), _vp, ctypes.POINTER(_vp),
+                       _f64, _i32, _vp, _sz, ctypes.POINTER(LymSolveStats), _vp]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load liblym.so and bind every entry point (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is not built: run `python -m tfleonardym_amd.build` (hipcc, gfx950). "
+            "There is no CPU fallback for the Wilson operator.")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.lym_abi_version() != 1:
+        raise ImportError("liblym.so ABI version mismatch; rebuild")
+    _lib = lib
+    return lib
+
+
+def lib():
+    return load()
+
+
+def check(rc, what="liblym"):
+    """Map a C status to Python: <0 raises RuntimeError, >=0 is returned."""
+    if rc < 0:
+        msg = load().lym_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what}: {msg} (code {rc})")
+    return rc
+
+
+def require_device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("the Wilson-fermion kernels need a HIP device (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+_scratch = {}
+
+
+def reduce_scratch(device):
+    """Small per-device scratch tensor for lym_dot (reused across calls on the
+    current stream)."""
+    key = (device.index, torch.cuda.current_stream(device).cuda_stream)
+    t = _scratch.get(key)
+    if t is None:
+        n = int(load().lym_reduce_scratch_bytes(0))
+        t = torch.zeros(n + 512, dtype=torch.uint8, device=device)
+        _scratch[key] = t
+    return t

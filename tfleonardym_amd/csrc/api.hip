@@ -1,0 +1,244 @@
+// extern "C" entry points of liblym.so (declared in include/lym.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "lym_internal.h"
+
+namespace lym {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return LYM_ERR_HIP;
+}
+
+int64_t field_elems(const OpDesc& d) { return (int64_t)d.ncomp * d.lat.V; }
+
+int make_desc(const lym_wilson* w, OpDesc* d) {
+  if (!w) {
+    set_error("null operator description");
+    return LYM_ERR_ARG;
+  }
+  if (w->ndim != 2 && w->ndim != 4) {
+    set_error("Number of dimensions " + std::to_string(w->ndim) + " not supported for fermions!");
+    return LYM_ERR_ARG;
+  }
+  if (w->nc != 2 && w->nc != 3) {
+    set_error("number of colours must be 2 or 3");
+    return LYM_ERR_UNSUPPORTED;
+  }
+  if (w->representation != LYM_REP_FUNDAMENTAL && w->representation != LYM_REP_ADJOINT) {
+    set_error("unknown representation");
+    return LYM_ERR_ARG;
+  }
+  OpDesc o;
+  o.ndim = w->ndim;
+  o.nc = w->nc;
+  o.adj = w->representation == LYM_REP_ADJOINT;
+  o.herm = w->hermitian ? 1 : 0;
+  o.ns = w->ndim == 4 ? 4 : 2;
+  o.ncc = o.adj ? w->nc * w->nc : w->nc;
+  o.ncomp = o.ns * o.ncc;
+  int64_t V = 1;
+  for (int mu = 0; mu < 4; ++mu) o.lat.L[mu] = 1;
+  for (int mu = 0; mu < w->ndim; ++mu) {
+    if (w->dims[mu] < 1) {
+      set_error("lattice extents must be positive");
+      return LYM_ERR_ARG;
+    }
+    o.lat.L[mu] = w->dims[mu];
+    V *= w->dims[mu];
+  }
+  if (V > (int64_t)1 << 31) {
+    set_error("lattice volume above 2^31 sites");
+    return LYM_ERR_ARG;
+  }
+  int64_t st = 1;
+  for (int mu = w->ndim - 1; mu >= 0; --mu) {
+    o.lat.stride[mu] = (int)st;
+    st *= w->dims[mu];
+  }
+  o.lat.V = V;
+  o.lat.kappa = w->kappa;
+  o.lat.bc_re = w->bc_re;
+  o.lat.bc_im = w->bc_im;
+  o.lat.herm = o.herm;
+  o.lat.has_bc = (w->bc_re != 1.0 || w->bc_im != 0.0) ? 1 : 0;
+  o.links64 = w->links;
+  o.linksr64 = w->links_right;
+  o.links32 = w->links_f32;
+  o.linksr32 = w->links_right_f32;
+  *d = o;
+  return LYM_OK;
+}
+
+size_t solver_workspace_bytes(const OpDesc& d, int prec, int nshift);
+int solve_cgm(const OpDesc& d, int prec, int nshift, const double* shifts, const void* b, void* const* xs, double tol,
+              int max_iter, void* ws, size_t ws_bytes, lym_solve_stats* stats, hipStream_t st);
+int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const void* b, void* const* xs,
+             int max_steps, double precision, double omega, void* ws, size_t ws_bytes, lym_solve_stats* stats,
+             hipStream_t st);
+
+}  // namespace lym
+
+using namespace lym;
+
+static size_t reduce_scratch(int64_t) {
+  // partials (kMaxRed x 2048) + ticket + results, each 256-aligned
+  return 256 * 4 + sizeof(double) * kMaxRed * 2048 + 64 + sizeof(double) * 16;
+}
+
+extern "C" {
+
+int lym_abi_version(void) { return LYM_ABI_VERSION; }
+
+int lym_init(int device) {
+  int cur = -1;
+  LYM_HIP(hipGetDevice(&cur));
+  if (cur != device) LYM_HIP(hipSetDevice(device));
+  return LYM_OK;
+}
+
+const char* lym_last_error(void) { return g_err.c_str(); }
+
+int lym_wilson_apply(const lym_wilson* op, int prec, int mode, const void* in, void* out, void* stream) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (!in || !out || in == out) {
+    set_error("lym_wilson_apply: in/out must be distinct non-null device pointers");
+    return LYM_ERR_ARG;
+  }
+  if (prec != LYM_PREC_F64 && prec != LYM_PREC_F32) {
+    set_error("lym_wilson_apply: precision must be F64 or F32");
+    return LYM_ERR_ARG;
+  }
+  EpiArgs a;
+  a.out.data = out;
+  Field f;
+  f.data = const_cast<void*>(in);
+  return wilson_launch(EPI_STORE, d, prec, mode != LYM_MODE_SIMPLE, f, a, RedCtl{nullptr, nullptr},
+                       (hipStream_t)stream);
+}
+
+int lym_wilson_apply_square(const lym_wilson* op, int prec, double shift, const void* in, void* out, void* tmp,
+                            void* stream) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (!in || !out || !tmp || in == out || in == tmp || out == tmp) {
+    set_error("lym_wilson_apply_square: in/out/tmp must be distinct non-null device pointers");
+    return LYM_ERR_ARG;
+  }
+  if (prec != LYM_PREC_F64 && prec != LYM_PREC_F32) {
+    set_error("lym_wilson_apply_square: precision must be F64 or F32");
+    return LYM_ERR_ARG;
+  }
+  EpiArgs a;
+  a.out.data = tmp;
+  Field f;
+  f.data = const_cast<void*>(in);
+  rc = wilson_launch(EPI_STORE, d, prec, true, f, a, RedCtl{nullptr, nullptr}, (hipStream_t)stream);
+  if (rc) return rc;
+  EpiArgs a2;
+  a2.out.data = out;
+  a2.aux0.data = const_cast<void*>(in);
+  a2.sigma = shift;
+  Field t;
+  t.data = tmp;
+  return wilson_launch(EPI_AXPY, d, prec, true, t, a2, RedCtl{nullptr, nullptr}, (hipStream_t)stream);
+}
+
+size_t lym_reduce_scratch_bytes(int64_t n) { return reduce_scratch(n); }
+
+int lym_dot(int prec, int64_t n, const void* x, const void* y, double* result, void* scratch, void* stream) {
+  if (!x || !y || !result || !scratch || n < 0) {
+    set_error("lym_dot: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  char* b = (char*)(((uintptr_t)scratch + 255) & ~(uintptr_t)255);
+  double* partials = (double*)b;
+  unsigned* ticket = (unsigned*)(b + (((sizeof(double) * kMaxRed * 2048) + 255) & ~(size_t)255));
+  double* res = (double*)((char*)ticket + 256);
+  hipStream_t st = (hipStream_t)stream;
+  if (n == 0) {
+    result[0] = result[1] = 0.0;
+    return LYM_OK;
+  }
+  int rc = blas_dot(prec, n, x, y, res, RedCtl{partials, ticket}, st);
+  if (rc) return rc;
+  LYM_HIP(hipMemcpyAsync(result, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  LYM_HIP(hipStreamSynchronize(st));
+  return LYM_OK;
+}
+
+int lym_axpby(int prec, int64_t n, double a_re, double a_im, const void* x, double b_re, double b_im, void* y,
+              void* stream) {
+  if (!x || !y || n < 0) {
+    set_error("lym_axpby: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  if (n == 0) return LYM_OK;
+  return blas_axpby(prec, n, a_re, a_im, x, b_re, b_im, y, (hipStream_t)stream);
+}
+
+int lym_gamma5(int prec, int ns, int64_t n_per_spin, const void* in, void* out, void* stream) {
+  if (!in || !out || ns < 2 || (ns & 1) || n_per_spin < 0) {
+    set_error("lym_gamma5: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  if (n_per_spin == 0) return LYM_OK;
+  return blas_gamma5(prec, ns, n_per_spin, in, out, (hipStream_t)stream);
+}
+
+int lym_convert(int prec_in, const void* in, int prec_out, void* out, int64_t n, void* stream) {
+  if (!in || !out || n < 0) {
+    set_error("lym_convert: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  if (n == 0) return LYM_OK;
+  return blas_convert(prec_in, in, prec_out, out, n, (hipStream_t)stream);
+}
+
+size_t lym_solver_workspace_bytes(const lym_wilson* op, int prec, int nshift) {
+  OpDesc d;
+  if (make_desc(op, &d)) return 0;
+  if (nshift < 1 || nshift > kMaxShifts) return 0;
+  return solver_workspace_bytes(d, prec, nshift);
+}
+
+int lym_mr_multishift(const lym_wilson* op, int nshift, const double* shifts, int first, const void* b,
+                      void* const* xs, int max_steps, double precision, double omega, void* workspace,
+                      size_t workspace_bytes, lym_solve_stats* stats, void* stream) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (!b || !xs || !shifts) {
+    set_error("lym_mr_multishift: null argument");
+    return LYM_ERR_ARG;
+  }
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  return solve_mr(d, nshift, shifts, first, b, xs, max_steps, precision, omega, workspace, workspace_bytes, stats,
+                  (hipStream_t)stream);
+}
+
+int lym_cgm(const lym_wilson* op, int prec, int nshift, const double* shifts, const void* b, void* const* xs,
+            double tol, int max_iter, void* workspace, size_t workspace_bytes, lym_solve_stats* stats, void* stream) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (!b || !xs || !shifts || !(tol > 0.0) || max_iter < 1) {
+    set_error("lym_cgm: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  return solve_cgm(d, prec, nshift, shifts, b, xs, tol, max_iter, workspace, workspace_bytes, stats,
+                   (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+// BLAS-1 kernels over flat arrays of complex numbers (complex128 / complex64).
+// Reductions accumulate in FP64 with a fixed partition and order, so results
+// depend only on n and the data.
+#include <hip/hip_runtime.h>
+
+#include "lym_internal.h"
+
+namespace lym {
+
+template <typename E>
+struct ElemR;
+template <>
+struct ElemR<double2> {
+  using R = double;
+};
+template <>
+struct ElemR<float2> {
+  using R = float;
+};
+
+// Number of blocks for a grid-stride reduction over n elements: depends on n only.
+static int red_grid(int64_t n) {
+  int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
+  if (g > 2048) g = 2048;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+template <typename E>
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const E* __restrict__ x, const E* __restrict__ y,
+                                                RedCtl rc, double* out) {
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const E a = x[i], b = y[i];
+    v[0] += (double)a.x * b.x + (double)a.y * b.y;
+    v[1] += (double)a.x * b.y - (double)a.y * b.x;
+  }
+  double sums[2];
+  if (block_reduce_last<2>(v, rc, sums)) {
+    if (threadIdx.x == 0) {
+      out[0] = sums[0];
+      out[1] = sums[1];
+    }
+  }
+}
+
+// out[0] = sum |a - b|^2, out[1] = sum |a|^2
+template <typename E>
+__global__ __launch_bounds__(kBlock) void k_diff_norm2(int64_t n, const E* __restrict__ a, const E* __restrict__ b,
+                                                       RedCtl rc, double* out) {
+  double v[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const E p = a[i], q = b[i];
+    const double dr = (double)p.x - (double)q.x, di = (double)p.y - (double)q.y;
+    v[0] += dr * dr + di * di;
+    v[1] += (double)p.x * p.x + (double)p.y * p.y;
+  }
+  double sums[2];
+  if (block_reduce_last<2>(v, rc, sums)) {
+    if (threadIdx.x == 0) {
+      out[0] = sums[0];
+      out[1] = sums[1];
+    }
+  }
+}
+
+template <typename E>
+__global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double ar, double ai, const E* __restrict__ x, double br,
+                                                  double bi, E* __restrict__ y) {
+  using R = typename ElemR<E>::R;
+  const R Ar = (R)ar, Ai = (R)ai, Br = (R)br, Bi = (R)bi;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const E a = x[i];
+    E b = y[i];
+    const R re = Ar * a.x - Ai * a.y + Br * b.x - Bi * b.y;
+    const R im = Ar * a.y + Ai * a.x + Br * b.y + Bi * b.x;
+    b.x = re;
+    b.y = im;
+    y[i] = b;
+  }
+}
+
+template <typename EI, typename EO>
+__global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const EI* __restrict__ in, EO* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const EI a = in[i];
+    EO b;
+    b.x = a.x;
+    b.y = a.y;
+    out[i] = b;
+  }
+}
+
+template <typename E>
+__global__ __launch_bounds__(kBlock) void k_gamma5(int ns, int64_t nps, const E* __restrict__ in, E* __restrict__ out) {
+  const int64_t n = (int64_t)ns * nps;
+  const int64_t half = (int64_t)(ns / 2) * nps;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    E a = in[i];
+    if (i >= half) {
+      a.x = -a.x;
+      a.y = -a.y;
+    }
+    out[i] = a;
+  }
+}
+
+static int stream_grid(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+int blas_dot(int prec, int64_t n, const void* x, const void* y, double* out_dev, RedCtl rc, hipStream_t st) {
+  const int g = red_grid(n);
+  if (prec == P64)
+    hipLaunchKernelGGL(k_dot<double2>, dim3(g), dim3(kBlock), 0, st, n, (const double2*)x, (const double2*)y, rc,
+                       out_dev);
+  else if (prec == P32)
+    hipLaunchKernelGGL(k_dot<float2>, dim3(g), dim3(kBlock), 0, st, n, (const float2*)x, (const float2*)y, rc,
+                       out_dev);
+  else {
+    set_error("lym_dot: precision must be F64 or F32");
+    return LYM_ERR_ARG;
+  }
+  LYM_HIP(hipGetLastError());
+  return LYM_OK;
+}
+
+int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* out_dev, RedCtl rc, hipStream_t st) {
+  const int g = red_grid(n);
+  if (prec == P64)
+    hipLaunchKernelGGL(k_diff_norm2<double2>, dim3(g), dim3(kBlock), 0, st, n, (const double2*)a, (const double2*)b,
+                       rc, out_dev);
+  else
+    hipLaunchKernelGGL(k_diff_norm2<float2>, dim3(g), dim3(kBlock), 0, st, n, (const float2*)a, (const float2*)b, rc,
+                       out_dev);
+  LYM_HIP(hipGetLastError());
+  return LYM_OK;
+}
+
+int blas_axpby(int prec, int64_t n, double ar, double ai, const void* x, double br, double bi, void* y,
+               hipStream_t st) {
+  const int g = stream_grid(n);
+  if (prec == P64)
+    hipLaunchKernelGGL(k_axpby<double2>, dim3(g), dim3(kBlock), 0, st, n, ar, ai, (const double2*)x, br, bi,
+                       (double2*)y);
+  else if (prec == P32)
+    hipLaunchKernelGGL(k_axpby<float2>, dim3(g), dim3(kBlock), 0, st, n, ar, ai, (const float2*)x, br, bi,
+                       (float2*)y);
+  else {
+    set_error("lym_axpby: precision must be F64 or F32");
+    return LYM_ERR_ARG;
+  }
+  LYM_HIP(hipGetLastError());
+  return LYM_OK;
+}
+
+int blas_convert(int pin, const void* in, int pout, void* out, int64_t n, hipStream_t st) {
+  const int g = stream_grid(n);
+  if (pin == P64 && pout == P32)
+    hipLaunchKernelGGL((k_convert<double2, float2>), dim3(g), dim3(kBlock), 0, st, n, (const double2*)in, (float2*)out);
+  else if (pin == P32 && pout == P64)
+    hipLaunchKernelGGL((k_convert<float2, double2>), dim3(g), dim3(kBlock), 0, st, n, (const float2*)in, (double2*)out);
+  else if (pin == P64 && pout == P64)
+    hipLaunchKernelGGL((k_convert<double2, double2>), dim3(g), dim3(kBlock), 0, st, n, (const double2*)in,
+                       (double2*)out);
+  else if (pin == P32 && pout == P32)
+    hipLaunchKernelGGL((k_convert<float2, float2>), dim3(g), dim3(kBlock), 0, st, n, (const float2*)in, (float2*)out);
+  else {
+    set_error("lym_convert: precisions must be F64 or F32");
+    return LYM_ERR_ARG;
+  }
+  LYM_HIP(hipGetLastError());
+  return LYM_OK;
+}
+
+int blas_gamma5(int prec, int ns, int64_t nps, const void* in, void* out, hipStream_t st) {
+  const int g = stream_grid((int64_t)ns * nps);
+  if (prec == P64)
+    hipLaunchKernelGGL(k_gamma5<double2>, dim3(g), dim3(kBlock), 0, st, ns, nps, (const double2*)in, (double2*)out);
+  else if (prec == P32)
+    hipLaunchKernelGGL(k_gamma5<float2>, dim3(g), dim3(kBlock), 0, st, ns, nps, (const float2*)in, (float2*)out);
+  else {
+    set_error("lym_gamma5: precision must be F64 or F32");
+    return LYM_ERR_ARG;
+  }
+  LYM_HIP(hipGetLastError());
+  return LYM_OK;
+}
+
+}  // namespace lym

@@ -1,0 +1,67 @@
+// Host build of the per-site Wilson arithmetic used by the kernels
+// (wilson_site.h), looped over all sites.  Test-only: lets the CPU test suite
+// check site indexing, gamma tables, colour transport and the boundary factor
+// against the oracle without a GPU.  Not part of liblym.so.
+#include <hip/hip_runtime.h>
+
+#include "wilson_site.h"
+
+using namespace lym;
+
+template <int D, int NC, bool ADJ, bool PROJ>
+static void run(const Lat& lat, const double* links, const double* links_r, const double* in, double* out) {
+  constexpr int NCOMP = Gam<D>::NS * (ADJ ? NC * NC : NC);
+  SpinorView<P64> sin{(double2*)in, nullptr, lat.V};
+  SpinorView<P64> sout{(double2*)out, nullptr, lat.V};
+  LinkView<double> lk{(const cpx<double>*)links, lat.V};
+  LinkView<double> lkr{(const cpx<double>*)(links_r ? links_r : links), lat.V};
+  for (int64_t s = 0; s < lat.V; ++s) {
+    cpx<double> hop[NCOMP], c[NCOMP], res[NCOMP];
+    wilson_hop<D, NC, ADJ, PROJ, double>(lat, s, sin, lk, lkr, hop);
+    sin.template load<NCOMP>(s, c);
+    wilson_finish<NCOMP, double>(lat, c, hop, res);
+    sout.template store<NCOMP>(s, res);
+  }
+}
+
+template <int D, int NC, bool ADJ>
+static int run_mode(int proj, const Lat& lat, const double* l, const double* lr, const double* in, double* out) {
+  if (proj) run<D, NC, ADJ, true>(lat, l, lr, in, out);
+  else run<D, NC, ADJ, false>(lat, l, lr, in, out);
+  return 0;
+}
+
+extern "C" int lymcheck_wilson_apply(int ndim, int nc, int adj, int herm, const int* dims, double kappa, double bc_re,
+                                     double bc_im, int proj, const double* links, const double* links_r,
+                                     const double* in, double* out) {
+  Lat lat{};
+  int64_t V = 1;
+  for (int mu = 0; mu < 4; ++mu) lat.L[mu] = 1;
+  for (int mu = 0; mu < ndim; ++mu) {
+    lat.L[mu] = dims[mu];
+    V *= dims[mu];
+  }
+  int64_t st = 1;
+  for (int mu = ndim - 1; mu >= 0; --mu) {
+    lat.stride[mu] = (int)st;
+    st *= dims[mu];
+  }
+  lat.V = V;
+  lat.kappa = kappa;
+  lat.bc_re = bc_re;
+  lat.bc_im = bc_im;
+  lat.herm = herm;
+  lat.has_bc = (bc_re != 1.0 || bc_im != 0.0);
+  const int key = ndim * 100 + nc * 10 + adj;
+  switch (key) {
+    case 420: return run_mode<4, 2, false>(proj, lat, links, links_r, in, out);
+    case 430: return run_mode<4, 3, false>(proj, lat, links, links_r, in, out);
+    case 220: return run_mode<2, 2, false>(proj, lat, links, links_r, in, out);
+    case 230: return run_mode<2, 3, false>(proj, lat, links, links_r, in, out);
+    case 421: return run_mode<4, 2, true>(proj, lat, links, links_r, in, out);
+    case 431: return run_mode<4, 3, true>(proj, lat, links, links_r, in, out);
+    case 221: return run_mode<2, 2, true>(proj, lat, links, links_r, in, out);
+    case 231: return run_mode<2, 3, true>(proj, lat, links, links_r, in, out);
+  }
+  return -1;
+}

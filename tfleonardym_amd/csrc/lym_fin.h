@@ -1,0 +1,133 @@
+// Scalar recurrences run by thread 0 of the last block of a reducing kernel.
+#pragma once
+#include <math.h>
+
+#include "lym_state.h"
+
+namespace lym {
+
+// CG-M after <p, (A+s0) p> is known: step length alpha and the shifted
+// zeta_{k+1}, alpha^s_k (Jegerlehner hep-lat/9612014):
+//   zeta_{k+1} = zeta_k zeta_{k-1} a_{k-1} /
+//                ( a_k b_{k-1} (zeta_{k-1} - zeta_k) + zeta_{k-1} a_{k-1} (1 + d_s a_k) )
+//   alpha^s_k  = a_k zeta_{k+1} / zeta_k
+__device__ inline void cgm_fin_a(SolverState* st, double tt, double pp) {
+  const double pAp = tt + st->sigma0 * pp;
+  if (!(pAp > 0.0) || !isfinite(pAp)) {
+    st->status = ST_BREAKDOWN;
+    st->phase = PH_DONE;
+    return;
+  }
+  const double alpha = st->rr / pAp;
+  st->pAp = pAp;
+  st->alpha = alpha;
+  const double ao = st->alpha_old, bo = st->beta_old;
+  for (int s = 0; s < st->nshift; ++s) {
+    if (!st->active[s]) continue;
+    const double z = st->zeta[s], zo = st->zeta_old[s];
+    const double den = alpha * bo * (zo - z) + zo * ao * (1.0 + st->delta[s] * alpha);
+    const double zn = z * zo * ao / den;
+    st->zeta_new[s] = zn;
+    st->alpha_s[s] = alpha * zn / z;
+  }
+  st->phase = PH_K2;
+}
+
+// CG-M after <r_{k+1}, r_{k+1}> is known: beta, beta^s, roll the recurrence,
+// per-shift convergence (|zeta_s|^2 rr <= tol2_s), stop decision.
+__device__ inline void cgm_fin_b(SolverState* st, double rrn) {
+  const double beta = rrn / st->rr;
+  for (int s = 0; s < st->nshift; ++s) {
+    st->upd[s] = st->active[s];
+    if (!st->active[s]) continue;
+    const double q = st->zeta_new[s] / st->zeta[s];
+    st->beta_s[s] = beta * q * q;
+    st->zeta_old[s] = st->zeta[s];
+    st->zeta[s] = st->zeta_new[s];
+  }
+  st->alpha_old = st->alpha;
+  st->beta_old = beta;
+  st->rr = rrn;
+  st->iter += 1;
+  int nact = 0;
+  for (int s = 0; s < st->nshift; ++s) {
+    if (!st->active[s] || s == st->base) continue;
+    if (st->zeta[s] * st->zeta[s] * rrn <= st->tol2_s[s]) {
+      st->active[s] = 0;
+      st->conv_iter[s] = st->iter;
+    } else {
+      ++nact;
+    }
+  }
+  // the base carries the common Krylov space: it stops only after all others
+  if (st->active[st->base]) {
+    if (nact == 0 && rrn <= st->tol2_s[st->base]) {
+      st->active[st->base] = 0;
+      st->conv_iter[st->base] = st->iter;
+    } else {
+      ++nact;
+    }
+  }
+  st->nactive = nact;
+  if (!isfinite(rrn)) {
+    st->status = ST_BREAKDOWN;
+    st->final_k3 = 1;
+  } else if (nact == 0) {
+    st->status = ST_CONVERGED;
+    st->final_k3 = 1;
+  } else if (st->iter >= st->max_iter) {
+    st->status = ST_MAXITER;
+    st->final_k3 = 1;
+  }
+  st->phase = PH_K3;
+}
+
+// Multishift MR (dirac_solver.py:35-43): alpha = omega <p,r>/<p,p>;
+// f_i <- f_i / (1 + (s_i - s_N) alpha); x update coefficients c_i = alpha f_i.
+__device__ inline void mr_fin(SolverState* st, double pr_re, double pr_im, double pp) {
+  if (!(pp > 0.0) || !isfinite(pp)) {
+    st->status = ST_BREAKDOWN;
+    st->phase = PH_DONE;
+    return;
+  }
+  const double ar = st->omega * pr_re / pp, ai = st->omega * pr_im / pp;
+  st->mr_alpha_re = ar;
+  st->mr_alpha_im = ai;
+  const int n = st->nshift;
+  const double sN = st->shift[n - 1];
+  for (int i = st->base; i < n - 1; ++i) {
+    // d = 1 + (s_i - s_N) alpha ; f /= d
+    const double ds = st->shift[i] - sN;
+    const double dr = 1.0 + ds * ar, di = ds * ai;
+    const double den = dr * dr + di * di;
+    const double fr = st->f_re[i], fi = st->f_im[i];
+    const double nr = (fr * dr + fi * di) / den, ni = (fi * dr - fr * di) / den;
+    st->f_re[i] = nr;
+    st->f_im[i] = ni;
+    st->c_re[i] = ar * nr - ai * ni;
+    st->c_im[i] = ar * ni + ai * nr;
+  }
+  st->c_re[n - 1] = ar;
+  st->c_im[n - 1] = ai;
+  st->phase = PH_K3;
+}
+
+// After r <- r - alpha p: stop when |<r,r>| < precision (dirac_solver.py:47-50).
+__device__ inline void mr_fin_rr(SolverState* st, double rr) {
+  st->rr = rr;
+  st->iter += 1;
+  if (!isfinite(rr)) {
+    st->status = ST_BREAKDOWN;
+    st->phase = PH_DONE;
+  } else if (fabs(rr) < st->tol2) {
+    st->status = ST_CONVERGED;
+    st->phase = PH_DONE;
+  } else if (st->iter >= st->max_iter) {
+    st->status = ST_MAXITER;
+    st->phase = PH_DONE;
+  } else {
+    st->phase = PH_K1;
+  }
+}
+
+}  // namespace lym

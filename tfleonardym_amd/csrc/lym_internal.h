@@ -1,0 +1,82 @@
+// Internal interfaces shared by the translation units of liblym.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/lym.h"
+#include "lym_reduce.h"
+#include "lym_state.h"
+#include "lym_types.h"
+
+namespace lym {
+
+// A spinor field argument: data planes plus (F16 storage only) per-site norms.
+struct Field {
+  void* data = nullptr;
+  float* norm = nullptr;
+};
+
+// Validated operator description.
+struct OpDesc {
+  int ndim = 0, nc = 0, adj = 0, herm = 0;
+  int ns = 0, ncc = 0, ncomp = 0;  // spins, colour components per spin, total
+  Lat lat{};
+  const void* links64 = nullptr;
+  const void* linksr64 = nullptr;
+  const void* links32 = nullptr;
+  const void* linksr32 = nullptr;
+};
+
+// Arguments of a fused Wilson epilogue (meaning depends on the epilogue).
+struct EpiArgs {
+  Field out;   // primary output
+  Field aux0;  // secondary input (center values)
+  Field aux1;  // secondary in/out
+  double sigma = 0.0;
+  SolverState* st = nullptr;
+};
+
+// error reporting (thread local message)
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+#define LYM_HIP(call)                                   \
+  do {                                                  \
+    hipError_t _e = (call);                             \
+    if (_e != hipSuccess) return ::lym::hip_fail(_e, #call); \
+  } while (0)
+
+int make_desc(const lym_wilson* w, OpDesc* d);
+int64_t field_elems(const OpDesc& d);  // complex numbers per spinor field
+
+// ---- Wilson kernel launchers (wilson.hip) --------------------------------
+enum EpiKind : int {
+  EPI_STORE = 0,     // out = Op in
+  EPI_AXPY = 1,      // out = Op in + sigma * aux0
+  EPI_CGM_K1 = 2,    // out = Op in; reduce |out|^2, |in|^2; CG-M finalizer A
+  EPI_CGM_K2 = 3,    // Ap = Op in + sigma0 aux0; aux1 -= alpha Ap; reduce |aux1|^2; finalizer B
+  EPI_MR_K1 = 4,     // out = Op in (phase gated)
+  EPI_MR_K2 = 5,     // out = Op in + sigma aux0; reduce <out,aux0>, <out,out>; MR finalizer
+};
+int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, const EpiArgs& a, RedCtl rc,
+                  hipStream_t stream);
+
+// ---- vector kernels (blas.hip / solvers.hip) ------------------------------
+int blas_dot(int prec, int64_t n, const void* x, const void* y, double* partial_out_dev, RedCtl rc,
+             hipStream_t st);
+int blas_axpby(int prec, int64_t n, double ar, double ai, const void* x, double br, double bi, void* y,
+               hipStream_t st);
+int blas_convert(int pin, const void* in, int pout, void* out, int64_t n, hipStream_t st);
+int blas_gamma5(int prec, int ns, int64_t nps, const void* in, void* out, hipStream_t st);
+// out_dev[0] = sum |a-b|^2, out_dev[1] = sum |a|^2
+int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* out_dev, RedCtl rc, hipStream_t st);
+
+// Largest number of blocks any reducing kernel launches for volume V.
+inline int64_t max_red_blocks(int64_t V) {
+  const int64_t g = grid_for(V);
+  return g > 2048 ? g : 2048;
+}
+constexpr int kMaxRed = 4;  // most values any kernel reduces
+
+}  // namespace lym

@@ -1,0 +1,51 @@
+// Device-resident solver state.  All Krylov scalars live here in FP64 and are
+// updated by the last block of each reducing kernel, so an iteration is a
+// fixed launch sequence with no host round trip.  `phase` is a small state
+// machine: a kernel whose phase does not match returns at once, so iterations
+// launched after convergence cost only their launch.
+#pragma once
+#include <stdint.h>
+
+namespace lym {
+
+constexpr int kMaxShifts = 64;
+
+enum Phase : int {
+  PH_K1 = 0,    // t = Q p
+  PH_K2 = 1,    // Ap, residual update
+  PH_K3 = 2,    // per-shift vector updates
+  PH_DONE = 3,
+};
+
+enum SolveStatus : int { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_BREAKDOWN = 3 };
+
+struct SolverState {
+  int phase;
+  int iter;
+  int max_iter;
+  int status;
+  int nshift;
+  int base;      // index of the base (smallest) shift
+  int nactive;
+  int final_k3;  // K3 of the last iteration must still run
+  double rr;     // <r, r> (base residual, iterated)
+  double bb;     // <b, b>
+  double tol2;   // stopping threshold on rr (absolute)
+  double alpha, alpha_old, beta, beta_old;
+  double sigma0;  // base shift
+  double omega;   // MR over-relaxation
+  double pAp;
+  double mr_alpha_re, mr_alpha_im;
+  double zeta[kMaxShifts], zeta_old[kMaxShifts], zeta_new[kMaxShifts];
+  double alpha_s[kMaxShifts], beta_s[kMaxShifts], delta[kMaxShifts];
+  double shift[kMaxShifts];
+  double tol2_s[kMaxShifts];  // per-shift threshold on zeta^2 rr
+  // MR multishift (complex): f_i and the x update coefficient alpha*f_i
+  double f_re[kMaxShifts], f_im[kMaxShifts];
+  double c_re[kMaxShifts], c_im[kMaxShifts];
+  int upd[kMaxShifts];     // update x_s / p_s in this iteration's K3
+  int active[kMaxShifts];  // still iterating
+  int conv_iter[kMaxShifts];
+};
+
+}  // namespace lym

@@ -1,0 +1,289 @@
+// Complex arithmetic, field storage views and lattice indexing for the
+// Wilson-fermion kernels.  Everything here is __host__ __device__ so the same
+// site arithmetic can be compiled into the CPU check library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+namespace lym {
+
+enum Prec : int { P64 = 0, P32 = 1, P16 = 2 };
+
+template <typename R>
+struct cpx {
+  R re, im;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> mk(R re, R im) {
+  cpx<R> c;
+  c.re = re;
+  c.im = im;
+  return c;
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> czero() {
+  return mk<R>(R(0), R(0));
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.re + b.re, a.im + b.im);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> operator-(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.re - b.re, a.im - b.im);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R>& operator+=(cpx<R>& a, cpx<R> b) {
+  a.re += b.re;
+  a.im += b.im;
+  return a;
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R>& operator-=(cpx<R>& a, cpx<R> b) {
+  a.re -= b.re;
+  a.im -= b.im;
+  return a;
+}
+// a * b
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+// conj(a) * b
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> cmulc(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re);
+}
+// a * conj(b)
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> cmul_c(cpx<R> a, cpx<R> b) {
+  return mk<R>(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im);
+}
+// acc += a * b
+template <typename R>
+__host__ __device__ __forceinline__ void cfma(cpx<R>& acc, cpx<R> a, cpx<R> b) {
+  acc.re += a.re * b.re - a.im * b.im;
+  acc.im += a.re * b.im + a.im * b.re;
+}
+// acc += conj(a) * b
+template <typename R>
+__host__ __device__ __forceinline__ void cfmac(cpx<R>& acc, cpx<R> a, cpx<R> b) {
+  acc.re += a.re * b.re + a.im * b.im;
+  acc.im += a.re * b.im - a.im * b.re;
+}
+// acc += a * conj(b)
+template <typename R>
+__host__ __device__ __forceinline__ void cfma_c(cpx<R>& acc, cpx<R> a, cpx<R> b) {
+  acc.re += a.re * b.re + a.im * b.im;
+  acc.im += a.im * b.re - a.re * b.im;
+}
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> cscale(R s, cpx<R> a) {
+  return mk<R>(s * a.re, s * a.im);
+}
+// multiply by i^k (k = 0..3) -- exact: only swaps and sign flips
+template <int K, typename R>
+__host__ __device__ __forceinline__ cpx<R> ipow(cpx<R> a) {
+  if constexpr ((K & 3) == 0) return a;
+  else if constexpr ((K & 3) == 1) return mk<R>(-a.im, a.re);
+  else if constexpr ((K & 3) == 2) return mk<R>(-a.re, -a.im);
+  else return mk<R>(a.im, -a.re);
+}
+
+// Element s of component plane `plane`: the plane base is uniform across the
+// wavefront and the per-lane part is a 32-bit byte offset, so the loads use a
+// scalar base address plus one 32-bit VGPR per site (fields < 4 GiB/plane).
+template <typename T>
+__host__ __device__ __forceinline__ T& plane_at(T* base, int64_t plane, int64_t V, int64_t s) {
+  char* p = (char*)(base + plane * V);
+  return *(T*)(p + (uint32_t)((uint32_t)s * (uint32_t)sizeof(T)));
+}
+
+// Device loads/stores of plane elements through raw buffer instructions: the
+// buffer resource (4 SGPRs) and the plane offset (soffset SGPR) are uniform,
+// the site offset is one 32-bit VGPR; accesses past `bytes` read 0 / are
+// dropped by the hardware range check.  Fields must be < 4 GiB.
+template <typename T>
+__device__ __forceinline__ T buf_ld(const T* base, uint32_t bytes, int64_t plane, int64_t V, int64_t s) {
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
+  const int voff = (int)((uint32_t)s * (uint32_t)sizeof(T));
+  const int soff = (int)((uint32_t)(plane * V) * (uint32_t)sizeof(T));
+  T out;
+  if constexpr (sizeof(T) == 16) {
+    auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    __builtin_memcpy(&out, &v, 16);
+  } else if constexpr (sizeof(T) == 8) {
+    auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    __builtin_memcpy(&out, &v, 8);
+  } else {
+    static_assert(sizeof(T) == 4, "buffer element size");
+    auto v = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+    __builtin_memcpy(&out, &v, 4);
+  }
+  return out;
+}
+
+template <typename T>
+__device__ __forceinline__ void buf_st(T* base, uint32_t bytes, int64_t plane, int64_t V, int64_t s, const T& val) {
+  const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
+  const int voff = (int)((uint32_t)s * (uint32_t)sizeof(T));
+  const int soff = (int)((uint32_t)(plane * V) * (uint32_t)sizeof(T));
+  if constexpr (sizeof(T) == 16) {
+    __attribute__((ext_vector_type(4))) unsigned int v;
+    __builtin_memcpy(&v, &val, 16);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
+  } else if constexpr (sizeof(T) == 8) {
+    __attribute__((ext_vector_type(2))) unsigned int v;
+    __builtin_memcpy(&v, &val, 8);
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+  } else {
+    static_assert(sizeof(T) == 4, "buffer element size");
+    unsigned int v;
+    __builtin_memcpy(&v, &val, 4);
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
+  }
+}
+
+// element load/store: buffer instructions in the device pass, plain C++ on the host
+template <typename T>
+__host__ __device__ __forceinline__ T ld(const T* base, uint32_t bytes, int64_t plane, int64_t V, int64_t s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return buf_ld<T>(base, bytes, plane, V, s);
+#else
+  (void)bytes;
+  return plane_at(base, plane, V, s);
+#endif
+}
+template <typename T>
+__host__ __device__ __forceinline__ void st(T* base, uint32_t bytes, int64_t plane, int64_t V, int64_t s, const T& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  buf_st<T>(base, bytes, plane, V, s, v);
+#else
+  (void)bytes;
+  plane_at(base, plane, V, s) = v;
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// Storage of spinor fields: component planes, site index fastest.
+//   component c of site s lives at  base[c * V + s]
+// P64: complex128 (interleaved re/im), P32: complex64, P16: half2 scaled by a
+// per-site float norm (value = half * norm[s]).
+// ---------------------------------------------------------------------------
+template <int P>
+struct Store;
+
+template <>
+struct Store<P64> {
+  using R = double;
+  using elem = double2;
+  static constexpr int bytes = 16;
+};
+template <>
+struct Store<P32> {
+  using R = float;
+  using elem = float2;
+  static constexpr int bytes = 8;
+};
+template <>
+struct Store<P16> {
+  using R = float;
+  using elem = __half2;
+  static constexpr int bytes = 4;
+};
+
+template <int P>
+struct SpinorView {
+  using R = typename Store<P>::R;
+  using E = typename Store<P>::elem;
+  E* data;
+  float* norm;  // P16 only
+  int64_t V;
+
+  template <int NCOMP>
+  __host__ __device__ __forceinline__ void load(int64_t s, cpx<R> (&v)[NCOMP]) const {
+    const uint32_t bytes = (uint32_t)(NCOMP * V * (int64_t)sizeof(E));
+    if constexpr (P == P16) {
+      const float n = ld<float>(norm, (uint32_t)(V * 4), 0, V, s);
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        const float2 f = __half22float2(ld<E>(data, bytes, c, V, s));
+        v[c] = mk<R>(f.x * n, f.y * n);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        const E e = ld<E>(data, bytes, c, V, s);
+        v[c] = mk<R>(e.x, e.y);
+      }
+    }
+  }
+  template <int NCOMP>
+  __host__ __device__ __forceinline__ void store(int64_t s, const cpx<R> (&v)[NCOMP]) const {
+    const uint32_t bytes = (uint32_t)(NCOMP * V * (int64_t)sizeof(E));
+    if constexpr (P == P16) {
+      float m = 0.f;
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        m = fmaxf(m, fmaxf(fabsf(v[c].re), fabsf(v[c].im)));
+      }
+      const float inv = m > 0.f ? 1.0f / m : 0.f;
+      st<float>(norm, (uint32_t)(V * 4), 0, V, s, m);
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        st<E>(data, bytes, c, V, s, __floats2half2_rn(v[c].re * inv, v[c].im * inv));
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        E e;
+        e.x = v[c].re;
+        e.y = v[c].im;
+        st<E>(data, bytes, c, V, s, e);
+      }
+    }
+  }
+};
+
+// Links: (D, Nc, Nc, V) complex, site fastest.  Link precision is P64 or P32.
+template <typename R>
+struct LinkView {
+  const cpx<R>* data;
+  int64_t V;
+  template <int NC>
+  __host__ __device__ __forceinline__ void load(int mu, int64_t s, cpx<R> (&u)[NC][NC]) const {
+    const uint32_t bytes = (uint32_t)((int64_t)(mu + 1) * NC * NC * V * (int64_t)sizeof(cpx<R>));
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) u[i][j] = ld<cpx<R>>(data, bytes, (int64_t)(mu * NC + i) * NC + j, V, s);
+  }
+};
+
+// Lattice description passed by value to every Wilson kernel.
+struct Lat {
+  int L[4];
+  int stride[4];
+  int64_t V;
+  double kappa;
+  double bc_re, bc_im;  // factor on U_{D-1}(x) at x_{D-1} = L_{D-1}-1
+  int herm;             // 1: multiply the result by gamma5
+  int has_bc;           // bc factor != 1
+};
+
+template <int D>
+__host__ __device__ __forceinline__ void coords(const Lat& lat, int64_t s, int (&x)[D]) {
+  uint32_t r = (uint32_t)s;
+#pragma unroll
+  for (int mu = D - 1; mu >= 1; --mu) {
+    const uint32_t L = (uint32_t)lat.L[mu];
+    const uint32_t q = r / L;
+    x[mu] = (int)(r - q * L);
+    r = q;
+  }
+  x[0] = (int)r;
+}
+
+}  // namespace lym
