@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--dslash-reps", type=int, default=50)
+    ap.add_argument("--baseline-steps", type=int, default=2,
+                    help="timed solves of the plain FP64 CG-M for the speed-up figure (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -102,6 +104,21 @@ def main():
     torch.cuda.synchronize()
     total = time.perf_counter() - t_start
     solves_per_s = args.steps / total
+
+    # ---- the plain FP64 multishift CG (textbook CG-M, one RHS) on the same solve
+    plain = None
+    if args.precision != "double" and args.baseline_steps > 0:
+        ps = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="double",
+                                            verbose=False)
+        ps.solve(A, bd, shifts)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        for _ in range(args.baseline_steps):
+            ps.solve(A, bd, shifts)
+        torch.cuda.synchronize()
+        plain = args.baseline_steps / (time.perf_counter() - tb)
+        print(f"[bench] plain FP64 CG-M: {plain:.2f} solves/s, {ps.last_stats['iterations']} it", file=sys.stderr,
+              flush=True)
 
     # ---- Wilson kernel bandwidth (HIP events on the launch stream) ----------
     desc = op.ensure_f32()
@@ -163,6 +180,8 @@ def main():
                    "shift_range": [W["lo"], W["hi"]], "precision": args.precision, "tolerance": 1e-10},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "iterations": stats["iterations"],
+        "plain_fp64_solves_per_s": plain,
+        "speedup_vs_plain_fp64": (solves_per_s / plain) if plain else None,
         "max_true_rel_resid": max(stats["true_rel_resid"]),
         "dslash_gbs": {"f64": res["f64"]["gbs"], "f32": f32["gbs"]},
         "roofline": {"kernel": "k_wilson f32 (D, one RHS)", "bound": "hbm", "achieved": f32["gbs"],

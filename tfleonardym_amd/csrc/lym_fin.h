@@ -82,6 +82,62 @@ __device__ inline void cgm_fin_b(SolverState* st, double rrn) {
   st->phase = PH_K3;
 }
 
+// Fast CG-M after <r_{k+1}, r_{k+1}>: as cgm_fin_b, but the shift updates of
+// this iteration are recorded as step blk_j of the current block instead of
+// being applied now; the block is flushed (PH_K3) when full or when the solve
+// stops, otherwise the next iteration starts with p = r + beta p (PH_KP).
+__device__ inline void cgm_fin_b_fast(SolverState* st, double rrn) {
+  const double beta = rrn / st->rr;
+  const int j = st->blk_j;
+  for (int s = 0; s < st->nshift; ++s) {
+    if (!st->active[s]) continue;
+    const double q = st->zeta_new[s] / st->zeta[s];
+    const double bs = beta * q * q;
+    st->h_alpha[j][s] = st->alpha_s[s];
+    st->h_zeta[j][s] = st->zeta_new[s];
+    st->h_beta[j][s] = bs;
+    st->n_upd[s] = j + 1;
+    st->zeta_old[s] = st->zeta[s];
+    st->zeta[s] = st->zeta_new[s];
+  }
+  st->alpha_old = st->alpha;
+  st->beta_old = beta;
+  st->rr = rrn;
+  st->iter += 1;
+  st->blk_j = j + 1;
+  int nact = 0;
+  for (int s = 0; s < st->nshift; ++s) {
+    if (!st->active[s] || s == st->base) continue;
+    if (st->zeta[s] * st->zeta[s] * rrn <= st->tol2_s[s]) {
+      st->active[s] = 0;
+      st->conv_iter[s] = st->iter;
+    } else {
+      ++nact;
+    }
+  }
+  if (st->active[st->base]) {
+    if (nact == 0 && rrn <= st->tol2_s[st->base]) {
+      st->active[st->base] = 0;
+      st->conv_iter[st->base] = st->iter;
+    } else {
+      ++nact;
+    }
+  }
+  st->nactive = nact;
+  if (!isfinite(rrn)) {
+    st->status = ST_BREAKDOWN;
+    st->final_k3 = 1;
+  } else if (nact == 0) {
+    st->status = ST_CONVERGED;
+    st->final_k3 = 1;
+  } else if (st->iter >= st->max_iter) {
+    st->status = ST_MAXITER;
+    st->final_k3 = 1;
+  }
+  if (rrn <= st->switch_rr) st->want_switch = 1;
+  st->phase = (st->final_k3 || st->blk_j >= st->mblk) ? PH_K3 : PH_KP;
+}
+
 // Multishift MR (dirac_solver.py:35-43): alpha = omega <p,r>/<p,p>;
 // f_i <- f_i / (1 + (s_i - s_N) alpha); x update coefficients c_i = alpha f_i.
 __device__ inline void mr_fin(SolverState* st, double pr_re, double pr_im, double pp) {

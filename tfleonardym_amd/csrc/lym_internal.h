@@ -36,6 +36,10 @@ struct EpiArgs {
   Field aux1;  // secondary in/out
   double sigma = 0.0;
   SolverState* st = nullptr;
+  // fast CG-M residual ring: nslot fields of `slot_elems` elements each
+  Field ring;
+  int64_t slot_elems = 0;
+  int nslot = 0;
 };
 
 // error reporting (thread local message)
@@ -58,6 +62,7 @@ enum EpiKind : int {
   EPI_CGM_K2 = 3,    // Ap = Op in + sigma0 aux0; aux1 -= alpha Ap; reduce |aux1|^2; finalizer B
   EPI_MR_K1 = 4,     // out = Op in (phase gated)
   EPI_MR_K2 = 5,     // out = Op in + sigma aux0; reduce <out,aux0>, <out,out>; MR finalizer
+  EPI_CGM_K2F = 6,   // fast CG-M: ring[k+1] = ring[k] - alpha (Op in + sigma0 aux0); deferred finalizer
 };
 int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, const EpiArgs& a, RedCtl rc,
                   hipStream_t stream);

@@ -9,21 +9,25 @@
 namespace lym {
 
 constexpr int kMaxShifts = 64;
+constexpr int kMaxBlock = 16;  // deferred multishift updates: iterations per block
 
 enum Phase : int {
   PH_K1 = 0,    // t = Q p
   PH_K2 = 1,    // Ap, residual update
-  PH_K3 = 2,    // per-shift vector updates
+  PH_K3 = 2,    // per-shift vector updates (plain) / deferred block flush (fast)
   PH_DONE = 3,
+  PH_KP = 4,    // fast path: p = r + beta p
 };
 
 enum SolveStatus : int { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_BREAKDOWN = 3 };
 
 struct SolverState {
+  // the first four ints are what the host polls between chunks
   int phase;
   int iter;
-  int max_iter;
+  int want_switch;  // fast CG-M: base residual below the precision-switch threshold
   int status;
+  int max_iter;
   int nshift;
   int base;      // index of the base (smallest) shift
   int nactive;
@@ -46,6 +50,17 @@ struct SolverState {
   int upd[kMaxShifts];     // update x_s / p_s in this iteration's K3
   int active[kMaxShifts];  // still iterating
   int conv_iter[kMaxShifts];
+  // ---- fast CG-M: deferred shift updates over blocks of `mblk` iterations.
+  // Step j of the current block used r^{k0+j+1} (ring slot (k0+j+1) % (mblk+1))
+  // and the scalars below; n_upd[s] = steps recorded for shift s.
+  int mblk;
+  int blk_j;         // steps recorded in the current block
+  int precision_phase;  // 0 = FP64 Krylov vectors, 1 = low precision
+  double switch_rr;  // absolute threshold on rr for want_switch
+  int n_upd[kMaxShifts];
+  double h_alpha[kMaxBlock][kMaxShifts];
+  double h_zeta[kMaxBlock][kMaxShifts];
+  double h_beta[kMaxBlock][kMaxShifts];
 };
 
 }  // namespace lym

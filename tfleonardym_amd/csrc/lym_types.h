@@ -157,7 +157,7 @@ __host__ __device__ __forceinline__ T ld(const T* base, uint32_t bytes, int64_t 
 #endif
 }
 template <typename T>
-__host__ __device__ __forceinline__ void st(T* base, uint32_t bytes, int64_t plane, int64_t V, int64_t s, const T& v) {
+__host__ __device__ __forceinline__ void put(T* base, uint32_t bytes, int64_t plane, int64_t V, int64_t s, const T& v) {
 #if defined(__HIP_DEVICE_COMPILE__)
   buf_st<T>(base, bytes, plane, V, s, v);
 #else
@@ -230,10 +230,10 @@ struct SpinorView {
         m = fmaxf(m, fmaxf(fabsf(v[c].re), fabsf(v[c].im)));
       }
       const float inv = m > 0.f ? 1.0f / m : 0.f;
-      st<float>(norm, (uint32_t)(V * 4), 0, V, s, m);
+      put<float>(norm, (uint32_t)(V * 4), 0, V, s, m);
 #pragma unroll
       for (int c = 0; c < NCOMP; ++c) {
-        st<E>(data, bytes, c, V, s, __floats2half2_rn(v[c].re * inv, v[c].im * inv));
+        put<E>(data, bytes, c, V, s, __floats2half2_rn(v[c].re * inv, v[c].im * inv));
       }
     } else {
 #pragma unroll
@@ -241,7 +241,7 @@ struct SpinorView {
         E e;
         e.x = v[c].re;
         e.y = v[c].im;
-        st<E>(data, bytes, c, V, s, e);
+        put<E>(data, bytes, c, V, s, e);
       }
     }
   }

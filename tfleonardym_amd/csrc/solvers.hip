@@ -172,6 +172,168 @@ __global__ __launch_bounds__(kBlock) void k_mr_k3(SolverState* st, Field r, Fiel
 }
 
 // ---------------------------------------------------------------------------
+// Fast CG-M kernels: residual ring + deferred shift updates
+// ---------------------------------------------------------------------------
+// ring slot 0 = b, live p = 0, x_s = 0, p_s = b (FP64); reduce |b|^2.
+template <int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_fast_init(SolverState* st, Field b, Field r0, Field plive,
+                                                      const Field* xs, const Field* ps, int nshift, int64_t V,
+                                                      RedCtl rc) {
+  const int64_t s = xcd_block() * kBlock + threadIdx.x;
+  double red[1] = {0.0};
+  if (s < V) {
+    cpx<double> v[NCOMP], z[NCOMP];
+    view<P64>(b, V).template load<NCOMP>(s, v);
+    view<P64>(r0, V).template store<NCOMP>(s, v);
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) {
+      z[c] = czero<double>();
+      red[0] += v[c].re * v[c].re + v[c].im * v[c].im;
+    }
+    view<P64>(plive, V).template store<NCOMP>(s, z);
+    for (int k = 0; k < nshift; ++k) {
+      view<P64>(ps[k], V).template store<NCOMP>(s, v);
+      view<P64>(xs[k], V).template store<NCOMP>(s, z);
+    }
+  }
+  double sums[1];
+  if (block_reduce_last<1>(red, rc, sums) && threadIdx.x == 0) {
+    st->bb = sums[0];
+    st->rr = sums[0];
+    for (int k = 0; k < st->nshift; ++k) st->tol2_s[k] *= sums[0];
+    st->switch_rr *= sums[0];
+    if (sums[0] > 0.0) {
+      st->phase = PH_KP;
+    } else {
+      st->status = ST_CONVERGED;
+      st->phase = PH_DONE;
+    }
+  }
+}
+
+template <int P>
+__device__ __forceinline__ SpinorView<P> ring_view(const Field& ring, int64_t slot_elems, int slot, int64_t V) {
+  using E = typename Store<P>::elem;
+  return SpinorView<P>{(E*)ring.data + slot * slot_elems, ring.norm ? ring.norm + slot * V : nullptr, V};
+}
+
+// p = r^k + beta_{k-1} p   (r^k in ring slot k % nslot)
+template <int P, int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_fast_kp(SolverState* st, Field ring, int64_t slot_elems, int nslot,
+                                                    Field p, int64_t V, unsigned* ticket) {
+  using R = typename Store<P>::R;
+  if (st->phase != PH_KP) return;
+  const int64_t s = xcd_block() * kBlock + threadIdx.x;
+  if (s < V) {
+    const SpinorView<P> rv = ring_view<P>(ring, slot_elems, st->iter % nslot, V);
+    const SpinorView<P> pv = view<P>(p, V);
+    const R b = (R)st->beta_old;
+    cpx<R> r[NCOMP], q[NCOMP];
+    rv.template load<NCOMP>(s, r);
+    pv.template load<NCOMP>(s, q);
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) q[c] = mk<R>(r[c].re + b * q[c].re, r[c].im + b * q[c].im);
+    pv.template store<NCOMP>(s, q);
+  }
+  if (block_is_last(ticket) && threadIdx.x == 0) st->phase = PH_K1;
+}
+
+// Deferred shift updates of one block: for every shift, apply its recorded
+// steps j < n_upd[s]:  x_s += a_j p_s ;  p_s = z_j r^{k0+j+1} + b_j p_s
+// (x_s, p_s in FP64; residuals read once per block from the ring).
+template <int PR, int NCOMP, int MB>
+__global__ __launch_bounds__(kBlock) void k_fast_k3(SolverState* st, Field ring, int64_t slot_elems, int nslot,
+                                                    const Field* xs, const Field* ps, int nshift, int64_t V,
+                                                    unsigned* ticket) {
+  using E = typename Store<PR>::elem;
+  if (st->phase != PH_K3) return;
+  const int64_t s = xcd_block() * kBlock + threadIdx.x;
+  const int nb = st->blk_j;
+  const int k0 = st->iter - nb;
+  if (s < V) {
+    const uint32_t fb = (uint32_t)(NCOMP * V * 16);
+    const uint32_t rb = (uint32_t)(NCOMP * V * (int64_t)sizeof(E));
+    float rn[MB];
+#pragma unroll
+    for (int j = 0; j < MB; ++j) {
+      rn[j] = 1.f;
+      if constexpr (PR == P16) {
+        if (j < nb) {
+          const int slot = (k0 + j + 1) % nslot;
+          rn[j] = ld<float>(ring.norm + slot * V, (uint32_t)(V * 4), 0, V, s);
+        }
+      }
+    }
+#pragma unroll 1
+    for (int c = 0; c < NCOMP; ++c) {
+      cpx<double> r[MB];
+#pragma unroll
+      for (int j = 0; j < MB; ++j) {
+        r[j] = czero<double>();
+        if (j < nb) {
+          const int slot = (k0 + j + 1) % nslot;
+          const E e = ld<E>((const E*)ring.data + slot * slot_elems, rb, c, V, s);
+          if constexpr (PR == P16) {
+            const float2 f = __half22float2(e);
+            r[j] = mk<double>((double)(f.x * rn[j]), (double)(f.y * rn[j]));
+          } else {
+            r[j] = mk<double>((double)e.x, (double)e.y);
+          }
+        }
+      }
+      for (int k = 0; k < nshift; ++k) {
+        const int n = st->n_upd[k];
+        if (n == 0) continue;
+        double2 x = ld<double2>((const double2*)xs[k].data, fb, c, V, s);
+        double2 p = ld<double2>((const double2*)ps[k].data, fb, c, V, s);
+#pragma unroll
+        for (int j = 0; j < MB; ++j) {
+          if (j < n) {
+            const double a = st->h_alpha[j][k], z = st->h_zeta[j][k], b = st->h_beta[j][k];
+            x.x += a * p.x;
+            x.y += a * p.y;
+            p.x = z * r[j].re + b * p.x;
+            p.y = z * r[j].im + b * p.y;
+          }
+        }
+        put<double2>((double2*)xs[k].data, fb, c, V, s, x);
+        put<double2>((double2*)ps[k].data, fb, c, V, s, p);
+      }
+    }
+  }
+  if (block_is_last(ticket) && threadIdx.x == 0) {
+    st->blk_j = 0;
+    for (int k = 0; k < st->nshift; ++k) st->n_upd[k] = 0;
+    st->phase = st->final_k3 ? PH_DONE : PH_KP;
+  }
+}
+
+// Precision switch at a block boundary: low-precision copies of the current
+// residual (ring slot iter % nslot) and of the live direction.
+template <int PL, int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_fast_switch(SolverState* st, Field ring64, Field ringlo,
+                                                        int64_t slot_elems, int nslot, Field p64, Field plo,
+                                                        int64_t V) {
+  const int64_t s = xcd_block() * kBlock + threadIdx.x;
+  if (s < V) {
+    const int slot = st->iter % nslot;
+    cpx<double> r[NCOMP], p[NCOMP];
+    ring_view<P64>(ring64, slot_elems, slot, V).template load<NCOMP>(s, r);
+    view<P64>(p64, V).template load<NCOMP>(s, p);
+    using R = typename Store<PL>::R;
+    cpx<R> rl[NCOMP], pl[NCOMP];
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) {
+      rl[c] = mk<R>((R)r[c].re, (R)r[c].im);
+      pl[c] = mk<R>((R)p[c].re, (R)p[c].im);
+    }
+    ring_view<PL>(ringlo, slot_elems, slot, V).template store<NCOMP>(s, rl);
+    view<PL>(plo, V).template store<NCOMP>(s, pl);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->precision_phase = 1;
+}
+
+// ---------------------------------------------------------------------------
 // flat FP64 helpers for true residuals and defect correction
 // ---------------------------------------------------------------------------
 // r = b - ax ; out[0] = |r|^2, out[1] = |b|^2
@@ -251,7 +413,14 @@ struct Parts {
   Field b_lo, r_lo, t_lo;  // low-precision copies (mixed solves)
   Field x_lo[kMaxShifts];
   Field p[kMaxShifts];     // search directions at the solve precision
+  // fast CG-M (mixed): residual rings, live base direction, FP64 shift directions
+  Field ring64, ring32;    // kFastBlock + 1 contiguous slots each
+  int64_t slot_elems = 0;
+  Field plive64, plive32;
+  Field ps64[kMaxShifts];
 };
+
+constexpr int kFastBlock = 8;  // iterations per deferred shift-update block
 
 // prec: precision of the Krylov vectors (P64 for FP64 solves, P32/P16 mixed)
 static size_t carve(const OpDesc& d, int prec, int nshift, char* base, Parts* pt) {
@@ -268,13 +437,23 @@ static size_t carve(const OpDesc& d, int prec, int nshift, char* base, Parts* pt
   P.t64 = take_field(c, d, P64);
   P.u64 = take_field(c, d, P64);
   P.v64 = take_field(c, d, P64);
-  if (prec != P64) {
-    P.b_lo = take_field(c, d, prec);
-    P.r_lo = take_field(c, d, prec);
-    P.t_lo = take_field(c, d, prec);
-    for (int k = 0; k < nshift; ++k) P.x_lo[k] = take_field(c, d, prec);
+  if (prec == P64) {
+    for (int k = 0; k < nshift; ++k) P.p[k] = take_field(c, d, P64);
+  } else {
+    // single-shift defect correction buffers
+    P.b_lo = take_field(c, d, P32);
+    P.r_lo = take_field(c, d, P32);
+    P.t_lo = take_field(c, d, P32);
+    P.x_lo[0] = take_field(c, d, P32);
+    P.p[0] = take_field(c, d, P32);
+    // fast CG-M
+    P.slot_elems = field_elems(d);
+    P.ring64.data = c.take(field_bytes(d, P64) * (kFastBlock + 1));
+    P.ring32.data = c.take(field_bytes(d, P32) * (kFastBlock + 1));
+    P.plive64 = take_field(c, d, P64);
+    P.plive32 = take_field(c, d, P32);
+    for (int k = 0; k < nshift; ++k) P.ps64[k] = take_field(c, d, P64);
   }
-  for (int k = 0; k < nshift; ++k) P.p[k] = take_field(c, d, prec);
   return c.off + 256;
 }
 
@@ -301,7 +480,7 @@ static int ensure_sync() {
     (void)hipEventDestroy(g_sync.ev[1]);
     g_sync.pinned = nullptr;
   }
-  LYM_HIP(hipHostMalloc((void**)&g_sync.pinned, 4 * sizeof(int), hipHostMallocDefault));
+  LYM_HIP(hipHostMalloc((void**)&g_sync.pinned, 16 * sizeof(int), hipHostMallocDefault));
   LYM_HIP(hipEventCreateWithFlags(&g_sync.ev[0], hipEventDisableTiming));
   LYM_HIP(hipEventCreateWithFlags(&g_sync.ev[1], hipEventDisableTiming));
   g_sync.device = dev;
@@ -440,6 +619,143 @@ static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Fi
   return LYM_OK;
 }
 
+// Fast multishift CG: blocks of kFastBlock iterations
+//   [ p = r + beta p ; t = Q p (|t|^2,|p|^2) ; r' = r - alpha (Q t + s0 p) (|r'|^2) ] x m
+//   then one deferred update of every shift's x_s, p_s (FP64) from the ring.
+// Krylov vectors are FP64 until want_switch (base rr <= switch_tol^2 |b|^2),
+// then `lowprec`.  xs are the caller's FP64 solutions.
+static int cgm_fast(const OpDesc& d, const Parts& P, Field b, void* const* xs, int nshift, const double* shifts,
+                    const double* tol_rel, int max_iter, double switch_tol, int lowprec, hipStream_t st,
+                    SolverState* out_state) {
+  SolverState h;
+  std::memset(&h, 0, sizeof(h));
+  int base = 0;
+  for (int k = 1; k < nshift; ++k)
+    if (shifts[k] < shifts[base]) base = k;
+  h.phase = PH_KP;
+  h.max_iter = max_iter;
+  h.nshift = nshift;
+  h.base = base;
+  h.sigma0 = shifts[base];
+  h.alpha_old = 1.0;
+  h.beta_old = 0.0;
+  h.mblk = kFastBlock;
+  h.switch_rr = switch_tol * switch_tol;
+  for (int k = 0; k < nshift; ++k) {
+    h.shift[k] = shifts[k];
+    h.delta[k] = shifts[k] - shifts[base];
+    h.zeta[k] = h.zeta_old[k] = 1.0;
+    h.active[k] = 1;
+    h.tol2_s[k] = tol_rel[k] * tol_rel[k];
+    h.conv_iter[k] = -1;
+  }
+  Field hx[kMaxShifts], hp[kMaxShifts];
+  for (int k = 0; k < nshift; ++k) {
+    hx[k].data = xs[k];
+    hp[k] = P.ps64[k];
+  }
+  LYM_HIP(hipMemcpyAsync(P.st, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  LYM_HIP(hipMemcpyAsync(P.xs_dev, hx, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
+  LYM_HIP(hipMemcpyAsync(P.ps_dev, hp, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
+  const int64_t V = d.lat.V;
+  const int grid = grid_for(V);
+  const int nslot = kFastBlock + 1;
+  int rc = with_ncomp(d.ncomp, [&](auto N) -> int {
+    constexpr int NC = decltype(N)::value;
+    hipLaunchKernelGGL((k_fast_init<NC>), dim3(grid), dim3(kBlock), 0, st, P.st, b, P.ring64, P.plive64, P.xs_dev,
+                       P.ps_dev, nshift, V, redctl(P));
+    LYM_HIP(hipGetLastError());
+    return LYM_OK;
+  });
+  if (rc) return rc;
+
+  int prec = P64;
+  auto enqueue_block = [&]() -> int {
+    const Field ring = prec == P64 ? P.ring64 : P.ring32;
+    const Field pl = prec == P64 ? P.plive64 : P.plive32;
+    const Field t = prec == P64 ? P.t64 : P.t_lo;
+    for (int j = 0; j < kFastBlock; ++j) {
+      int e = with_prec_ncomp(prec, d.ncomp, [&](auto PP, auto N) -> int {
+        constexpr int PV = decltype(PP)::value, NC = decltype(N)::value;
+        hipLaunchKernelGGL((k_fast_kp<PV, NC>), dim3(grid), dim3(kBlock), 0, st, P.st, ring, P.slot_elems, nslot, pl,
+                           V, P.ticket);
+        LYM_HIP(hipGetLastError());
+        return LYM_OK;
+      });
+      if (e) return e;
+      EpiArgs a1;
+      a1.out = t;
+      a1.st = P.st;
+      e = wilson_launch(EPI_CGM_K1, d, prec, true, pl, a1, redctl(P), st);
+      if (e) return e;
+      EpiArgs a2;
+      a2.aux0 = pl;
+      a2.st = P.st;
+      a2.ring = ring;
+      a2.slot_elems = P.slot_elems;
+      a2.nslot = nslot;
+      e = wilson_launch(EPI_CGM_K2F, d, prec, true, t, a2, redctl(P), st);
+      if (e) return e;
+    }
+    return with_prec_ncomp(prec, d.ncomp, [&](auto PP, auto N) -> int {
+      constexpr int PV = decltype(PP)::value, NC = decltype(N)::value;
+      hipLaunchKernelGGL((k_fast_k3<PV, NC, kFastBlock>), dim3(grid), dim3(kBlock), 0, st, P.st, ring, P.slot_elems,
+                         nslot, P.xs_dev, P.ps_dev, nshift, V, P.ticket);
+      LYM_HIP(hipGetLastError());
+      return LYM_OK;
+    });
+  };
+  auto enqueue_switch = [&]() -> int {
+    return with_ncomp(d.ncomp, [&](auto N) -> int {
+      constexpr int NC = decltype(N)::value;
+      if (lowprec == P32) {
+        hipLaunchKernelGGL((k_fast_switch<P32, NC>), dim3(grid), dim3(kBlock), 0, st, P.st, P.ring64, P.ring32,
+                           P.slot_elems, nslot, P.plive64, P.plive32, V);
+      } else {
+        set_error("fast CG-M: unsupported low precision");
+        return (int)LYM_ERR_UNSUPPORTED;
+      }
+      LYM_HIP(hipGetLastError());
+      return LYM_OK;
+    });
+  };
+
+  rc = ensure_sync();
+  if (rc) return rc;
+  const int64_t max_blocks = (int64_t)max_iter / kFastBlock + 8;
+  int64_t issued = 0, polled = 0;
+  auto issue = [&](int slot) -> int {
+    int e = enqueue_block();
+    if (e) return e;
+    LYM_HIP(hipMemcpyAsync(&g_sync.pinned[4 * slot], P.st, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    LYM_HIP(hipEventRecord(g_sync.ev[slot], st));
+    ++issued;
+    return LYM_OK;
+  };
+  if ((rc = issue(0))) return rc;
+  if ((rc = issue(1))) return rc;
+  while (true) {
+    const int slot = (int)(polled & 1);
+    LYM_HIP(hipEventSynchronize(g_sync.ev[slot]));
+    ++polled;
+    const int* s4 = &g_sync.pinned[4 * slot];
+    if (s4[0] == PH_DONE) break;
+    if (issued > max_blocks) {
+      set_error("fast CG-M did not reach its stop state (internal error)");
+      return LYM_ERR_HIP;
+    }
+    if (s4[2] && prec == P64 && lowprec != P64) {
+      if ((rc = enqueue_switch())) return rc;
+      prec = lowprec;
+    }
+    if ((rc = issue(slot))) return rc;
+  }
+  LYM_HIP(hipStreamSynchronize(st));
+  LYM_HIP(hipMemcpyAsync(out_state, P.st, sizeof(SolverState), hipMemcpyDeviceToHost, st));
+  LYM_HIP(hipStreamSynchronize(st));
+  return LYM_OK;
+}
+
 static void fill_stats(lym_solve_stats* s, const SolverState& h, int nshift) {
   if (!s) return;
   s->status = h.status;
@@ -490,21 +806,16 @@ int solve_cgm(const OpDesc& d, int prec, int nshift, const double* shifts, const
     rc = cgm_core(d, P64, P, bf, xf, P.r64, P.t64, nshift, shifts, tol_rel.data(), max_iter, st, &h);
     if (rc) return rc;
   } else {
-    // mixed: low-precision CG-M to tol_lo, then FP64 defect correction per shift
-    const double tol_lo = std::max(tol, inner == P32 ? 1e-6 : 2e-3);
-    for (auto& v : tol_rel) v = tol_lo;
+    // mixed: FP64 Krylov vectors until the base residual is below 1e-4 of |b|,
+    // low precision after; shift solutions and directions accumulate in FP64
+    // with deferred block updates.  FP64 true residuals are checked below.
     if (inner == P16) {
       set_error("half-precision CG-M is not built yet");
       return LYM_ERR_UNSUPPORTED;
     }
-    rc = blas_convert(P64, b, P32, P.b_lo.data, n, st);
+    for (auto& v : tol_rel) v = 0.5 * tol;
+    rc = cgm_fast(d, P, bf, xs, nshift, shifts, tol_rel.data(), max_iter, 1e-4, inner, st, &h);
     if (rc) return rc;
-    rc = cgm_core(d, inner, P, P.b_lo, P.x_lo, P.r_lo, P.t_lo, nshift, shifts, tol_rel.data(), max_iter, st, &h);
-    if (rc) return rc;
-    for (int k = 0; k < nshift; ++k) {
-      rc = blas_convert(P32, P.x_lo[k].data, P64, xs[k], n, st);
-      if (rc) return rc;
-    }
   }
   // FP64 true residuals, and defect correction where needed
   const int max_refine = 12;

@@ -118,6 +118,53 @@ struct EpiCgmK2 {
   __device__ void finalize(const double* sums) const { cgm_fin_b(st, sums[0]); }
 };
 
+// Fast CG-M step 2: residuals live in a ring of nslot fields; iteration k reads
+// r^k from slot k % nslot and writes r^{k+1} to slot (k+1) % nslot, so the
+// last block's worth of residuals stays available for the deferred shift
+// updates.  r^{k+1} = r^k - alpha (Q t + s0 p); reduce |r^{k+1}|^2.
+template <int P, int NCOMP>
+struct EpiCgmK2F {
+  using R = typename Store<P>::R;
+  using E = typename Store<P>::elem;
+  static constexpr int NRED = 1;
+  SpinorView<P> p;
+  E* ring;
+  float* ring_norm;
+  int64_t slot_elems;
+  int nslot;
+  SolverState* st;
+  EpiCgmK2F(const EpiArgs& a, int64_t V)
+      : p{(E*)a.aux0.data, a.aux0.norm, V},
+        ring((E*)a.ring.data),
+        ring_norm(a.ring.norm),
+        slot_elems(a.slot_elems),
+        nslot(a.nslot),
+        st(a.st) {}
+  __device__ bool active() const { return st->phase == PH_K2; }
+  __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
+                        double* red) const {
+    const int k = st->iter;
+    const int si = k % nslot, so = (k + 1) % nslot;
+    const SpinorView<P> rin{ring + si * slot_elems, ring_norm ? ring_norm + si * lat.V : nullptr, lat.V};
+    const SpinorView<P> rout{ring + so * slot_elems, ring_norm ? ring_norm + so * lat.V : nullptr, lat.V};
+    cpx<R> c[NCOMP], res[NCOMP], pv[NCOMP], rv[NCOMP];
+    in.template load<NCOMP>(s, c);
+    p.template load<NCOMP>(s, pv);
+    rin.template load<NCOMP>(s, rv);
+    wilson_finish<NCOMP, R>(lat, c, hop, res);
+    const R s0 = (R)st->sigma0;
+    const R al = (R)st->alpha;
+#pragma unroll
+    for (int q = 0; q < NCOMP; ++q) {
+      const R apr = res[q].re + s0 * pv[q].re, api = res[q].im + s0 * pv[q].im;
+      rv[q] = mk<R>(rv[q].re - al * apr, rv[q].im - al * api);
+    }
+    rout.template store<NCOMP>(s, rv);
+    red[0] += norm2<NCOMP, R>(rv);
+  }
+  __device__ void finalize(const double* sums) const { cgm_fin_b_fast(st, sums[0]); }
+};
+
 // MR step 1: t = Op r (phase gated; the dummy reduction advances the phase).
 template <int P, int NCOMP>
 struct EpiMrK1 {
@@ -281,6 +328,7 @@ int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, 
     case EPI_AXPY: return launch_prec<EpiAxpy>(prec, d, proj, in, a, rc, stream);
     case EPI_CGM_K1: return launch_prec<EpiCgmK1>(prec, d, proj, in, a, rc, stream);
     case EPI_CGM_K2: return launch_prec<EpiCgmK2>(prec, d, proj, in, a, rc, stream);
+    case EPI_CGM_K2F: return launch_prec<EpiCgmK2F>(prec, d, proj, in, a, rc, stream);
     case EPI_MR_K1:
     case EPI_MR_K2:
       if (prec != P64) {
