@@ -35,13 +35,8 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const E* __restrict__
     v[0] += (double)a.x * b.x + (double)a.y * b.y;
     v[1] += (double)a.x * b.y - (double)a.y * b.x;
   }
-  double sums[2];
-  if (block_reduce_last<2>(v, rc, sums)) {
-    if (threadIdx.x == 0) {
-      out[0] = sums[0];
-      out[1] = sums[1];
-    }
-  }
+  block_partials<2>(v, rc.partials);
+  (void)out;
 }
 
 // out[0] = sum |a - b|^2, out[1] = sum |a|^2
@@ -55,13 +50,8 @@ __global__ __launch_bounds__(kBlock) void k_diff_norm2(int64_t n, const E* __res
     v[0] += dr * dr + di * di;
     v[1] += (double)p.x * p.x + (double)p.y * p.y;
   }
-  double sums[2];
-  if (block_reduce_last<2>(v, rc, sums)) {
-    if (threadIdx.x == 0) {
-      out[0] = sums[0];
-      out[1] = sums[1];
-    }
-  }
+  block_partials<2>(v, rc.partials);
+  (void)out;
 }
 
 template <typename E>
@@ -125,7 +115,7 @@ int blas_dot(int prec, int64_t n, const void* x, const void* y, double* out_dev,
     return LYM_ERR_ARG;
   }
   LYM_HIP(hipGetLastError());
-  return LYM_OK;
+  return launch_finalize(FK_SUM, nullptr, rc.partials, g, 2, out_dev, st);
 }
 
 int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* out_dev, RedCtl rc, hipStream_t st) {
@@ -137,7 +127,7 @@ int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* o
     hipLaunchKernelGGL(k_diff_norm2<float2>, dim3(g), dim3(kBlock), 0, st, n, (const float2*)a, (const float2*)b, rc,
                        out_dev);
   LYM_HIP(hipGetLastError());
-  return LYM_OK;
+  return launch_finalize(FK_SUM, nullptr, rc.partials, g, 2, out_dev, st);
 }
 
 int blas_axpby(int prec, int64_t n, double ar, double ai, const void* x, double br, double bi, void* y,

@@ -67,6 +67,22 @@ enum EpiKind : int {
 int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, const EpiArgs& a, RedCtl rc,
                   hipStream_t stream);
 
+// ---- finalizer kernel (solvers.hip): one block reduces the partials of the
+// preceding kernel in a fixed order, then advances the solver state.
+enum FinKind : int {
+  FK_SUM = 0,         // out[0..nr) = grid sums (no state)
+  FK_INIT = 1,        // bb = rr = sum; tolerances scaled by bb; DONE if bb == 0
+  FK_CGM_A = 2,       // gate K1|KP: step length and shifted zetas
+  FK_CGM_B = 3,       // gate K2: beta, roll, convergence (plain CG-M)
+  FK_CGM_BF = 4,      // gate K2: as FK_CGM_B, deferred-update recording (fast)
+  FK_CGM_K3ADV = 5,   // gate K3: phase -> K1 or DONE
+  FK_FAST_K3ADV = 6,  // gate K3: clear block, phase -> KP or DONE
+  FK_MR = 7,          // gate K1: MR alpha and shift coefficients
+  FK_MR_RR = 8,       // gate K3: MR stop test
+};
+int launch_finalize(FinKind kind, SolverState* st, const double* partials, int G, int nr, double* out,
+                    hipStream_t stream);
+
 // ---- vector kernels (blas.hip / solvers.hip) ------------------------------
 int blas_dot(int prec, int64_t n, const void* x, const void* y, double* partial_out_dev, RedCtl rc,
              hipStream_t st);

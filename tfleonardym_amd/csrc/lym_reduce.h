@@ -2,9 +2,11 @@
 //
 // wave_sum: DPP row butterflies (quad_perm, row_ror) inside each 16-lane row,
 // then two cross-row exchanges; every lane ends with the wave total.
-// block_reduce_last: per-block partials written in block order, an atomic
-// ticket elects the last block, which sums the partials in a fixed order; the
-// result never depends on which block finished last or on launch timing.
+// Grid reductions are two-pass: every block writes its partial sums in block
+// order (block_partials), and a one-block finalizer kernel adds them in a
+// fixed order (reduce_partials).  No fences or atomics inside the producing
+// kernel (on a multi-XCD part an agent-scope fence per block writes back L2),
+// and the result depends only on the data and the grid size.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -49,23 +51,16 @@ inline int grid_for(int64_t n, int per_block = kBlock) {
 }
 
 struct RedCtl {
-  double* partials;  // [NR][gridDim.x]
-  unsigned* ticket;  // zero between launches
+  double* partials;  // [NR][gridDim.x] written by the producing kernel
+  unsigned* ticket;  // unused by the two-pass scheme (kept for layout)
 };
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Reduces v[NR] over the grid.  Returns true in the last block, where
-// sums[NR] (shared memory) holds the grid totals for every thread.
+// Block sums of v[NR] -> partials[r * gridDim.x + blockIdx.x] (thread 0 writes).
 template <int NR>
-__device__ __forceinline__ bool block_reduce_last(double (&v)[NR], const RedCtl& rc, double (&sums)[NR]) {
+__device__ __forceinline__ void block_partials(const double (&v)[NR], double* partials) {
   __shared__ double sm[NR][kBlock / 64];
-  __shared__ int is_last;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
-  constexpr int NW = kBlock / 64;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     const double t = wave_sum(v[r]);
@@ -77,22 +72,24 @@ __device__ __forceinline__ bool block_reduce_last(double (&v)[NR], const RedCtl&
     for (int r = 0; r < NR; ++r) {
       double t = 0.0;
 #pragma unroll
-      for (int i = 0; i < NW; ++i) t += sm[r][i];
-      rc.partials[(int64_t)r * gridDim.x + blockIdx.x] = t;
+      for (int i = 0; i < kBlock / 64; ++i) t += sm[r][i];
+      partials[(int64_t)r * gridDim.x + blockIdx.x] = t;
     }
-    __threadfence();
-    const unsigned prev = atomicAdd(rc.ticket, 1u);
-    is_last = (prev == gridDim.x - 1) ? 1 : 0;
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+}
+
+// One block of kBlock threads: sums[r] = sum_b partials[r * G + b], fixed order.
+// Every thread returns the totals.
+template <int NR>
+__device__ __forceinline__ void reduce_partials(const double* partials, int G, double (&sums)[NR]) {
+  __shared__ double sm[NR][kBlock / 64];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     double t = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) t += load_coherent(&rc.partials[(int64_t)r * gridDim.x + i]);
+    for (int i = threadIdx.x; i < G; i += kBlock) t += partials[(int64_t)r * G + i];
     t = wave_sum(t);
-    __syncthreads();
     if (lane == 0) sm[r][w] = t;
   }
   __syncthreads();
@@ -100,25 +97,9 @@ __device__ __forceinline__ bool block_reduce_last(double (&v)[NR], const RedCtl&
   for (int r = 0; r < NR; ++r) {
     double t = 0.0;
 #pragma unroll
-    for (int i = 0; i < NW; ++i) t += sm[r][i];
+    for (int i = 0; i < kBlock / 64; ++i) t += sm[r][i];
     sums[r] = t;
   }
-  if (threadIdx.x == 0) atomicExch(rc.ticket, 0u);
-  return true;
-}
-
-// Ticket only (no values): true in the block that finishes last.
-__device__ __forceinline__ bool block_is_last(unsigned* ticket) {
-  __shared__ int is_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned prev = atomicAdd(ticket, 1u);
-    is_last = (prev == gridDim.x - 1) ? 1 : 0;
-    if (is_last) atomicExch(ticket, 0u);
-  }
-  __syncthreads();
-  return is_last != 0;
 }
 
 }  // namespace lym

@@ -73,7 +73,8 @@ struct EpiCgmK1 {
   SpinorView<P> out;
   SolverState* st;
   EpiCgmK1(const EpiArgs& a, int64_t V) : out{(typename Store<P>::elem*)a.out.data, a.out.norm, V}, st(a.st) {}
-  __device__ bool active() const { return st->phase == PH_K1; }
+  // plain CG-M enters at PH_K1, the fast variant at PH_KP (after p = r + beta p)
+  __device__ bool active() const { return st->phase == PH_K1 || st->phase == PH_KP; }
   __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
                         double* red) const {
     cpx<R> c[NCOMP], res[NCOMP];
@@ -165,11 +166,11 @@ struct EpiCgmK2F {
   __device__ void finalize(const double* sums) const { cgm_fin_b_fast(st, sums[0]); }
 };
 
-// MR step 1: t = Op r (phase gated; the dummy reduction advances the phase).
+// MR step 1: t = Op r (phase gated).
 template <int P, int NCOMP>
 struct EpiMrK1 {
   using R = typename Store<P>::R;
-  static constexpr int NRED = 1;
+  static constexpr int NRED = 0;
   SpinorView<P> out;
   SolverState* st;
   EpiMrK1(const EpiArgs& a, int64_t V) : out{(typename Store<P>::elem*)a.out.data, a.out.norm, V}, st(a.st) {}
@@ -195,7 +196,7 @@ struct EpiMrK2 {
       : out{(typename Store<P>::elem*)a.out.data, a.out.norm, V},
         r{(typename Store<P>::elem*)a.aux0.data, a.aux0.norm, V},
         st(a.st) {}
-  __device__ bool active() const { return st->phase == PH_K2; }
+  __device__ bool active() const { return st->phase == PH_K1; }
   __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
                         double* red) const {
     cpx<R> c[NCOMP], res[NCOMP], rv[NCOMP];
@@ -235,12 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_wilson(Lat lat, SpinorView<P> in, Li
     wilson_hop<D, NC, ADJ, PROJ, R>(lat, s, in, lk, lkr, hop);
     epi.apply(lat, s, in, hop, red);
   }
-  if constexpr (Epi::NRED > 0) {
-    double sums[NR];
-    if (block_reduce_last<NR>(red, rc, sums)) {
-      if (threadIdx.x == 0) epi.finalize(sums);
-    }
-  }
+  if constexpr (Epi::NRED > 0) block_partials<NR>(red, rc.partials);
 }
 
 // ---------------------------------------------------------------------------
