@@ -55,6 +55,19 @@ def test_wilson_matches_oracle(geometry, nc, rep, herm, bc, mode):
     assert rel(got, want) < TOL64
 
 
+@pytest.mark.parametrize("geometry", [(8, 8, 8, 8), (8, 4, 4, 16), (16, 8, 8, 16)])
+@pytest.mark.parametrize("nc,rep", [(3, "fundamental"), (2, "adjoint"), (2, "fundamental")])
+@pytest.mark.parametrize("bc", [None, -1.0])
+def test_tiled_schedule_parity(geometry, nc, rep, bc):
+    """Lattices where the Wilson kernels use the XCD-tiled block schedule."""
+    print(f"case tiled geometry={geometry} nc={nc} rep={rep} bc={bc}", flush=True)
+    U = fields.gauge_field(geometry, nc, seed=33, kind="haar")
+    psi = fields.gaussian_spinor(geometry, 4, nc, seed=34, representation=rep)
+    want = ow.WilsonOracle(U, geometry, 0.125, True, rep, bc_factor=bc).apply(psi)
+    op = make_op(U, geometry, 0.125, True, rep, bc)
+    assert rel(op.apply(psi), want) < TOL64
+
+
 def test_custom_provider_is_materialised():
     """A provider that is not the standard one is applied once to the links."""
     from tfleonardym_amd import boundary_conditions, dirac, lattice

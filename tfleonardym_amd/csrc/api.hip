@@ -73,6 +73,30 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   o.linksr64 = w->links_right;
   o.links32 = w->links_f32;
   o.linksr32 = w->links_right_f32;
+  // block-resident field bytes must stay below 4 GiB (32-bit buffer offsets)
+  const int64_t link_bytes = (int64_t)w->ndim * w->nc * w->nc * V * 16;
+  const int64_t field_bytes = (int64_t)o.ncomp * V * 16;
+  if (link_bytes >= ((int64_t)1 << 32) || field_bytes >= ((int64_t)1 << 32)) {
+    set_error("lattice too large for one device field (>= 4 GiB per field)");
+    return LYM_ERR_ARG;
+  }
+  // Wilson block schedule: 4D tiled when the extents allow it, else linear
+  o.map = SiteMap{0, 0, 0, 0, 0};
+  o.wthreads = kBlock;
+  o.wblocks = grid_for(V);
+  if (w->ndim == 4) {
+    const int T = w->dims[3], L2 = w->dims[2], L0 = w->dims[0];
+    if (T <= kBlock && L0 % kNumXcd == 0) {
+      for (int k = kBlock / T; k >= 1; --k) {
+        if ((T * k) % 64 == 0 && L2 % k == 0) {
+          o.map = SiteMap{1, T, k, L0 / kNumXcd, L2 / k};
+          o.wthreads = T * k;
+          o.wblocks = (int)(V / (T * k));
+          break;
+        }
+      }
+    }
+  }
   *d = o;
   return LYM_OK;
 }

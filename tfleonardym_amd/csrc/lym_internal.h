@@ -23,6 +23,9 @@ struct OpDesc {
   int ndim = 0, nc = 0, adj = 0, herm = 0;
   int ns = 0, ncc = 0, ncomp = 0;  // spins, colour components per spin, total
   Lat lat{};
+  SiteMap map{};     // block schedule of the Wilson kernels
+  int wblocks = 0;   // Wilson grid size (blocks)
+  int wthreads = 0;  // Wilson block size (threads, multiple of 64)
   const void* links64 = nullptr;
   const void* linksr64 = nullptr;
   const void* links32 = nullptr;
@@ -93,9 +96,10 @@ int blas_gamma5(int prec, int ns, int64_t nps, const void* in, void* out, hipStr
 // out_dev[0] = sum |a-b|^2, out_dev[1] = sum |a|^2
 int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* out_dev, RedCtl rc, hipStream_t st);
 
-// Largest number of blocks any reducing kernel launches for volume V.
+// Largest number of blocks any reducing kernel launches for volume V
+// (Wilson blocks have at least 64 threads).
 inline int64_t max_red_blocks(int64_t V) {
-  const int64_t g = grid_for(V);
+  const int64_t g = V / 64 + kNumXcd + 1;
   return g > 2048 ? g : 2048;
 }
 constexpr int kMaxRed = 4;  // most values any kernel reduces

@@ -56,11 +56,13 @@ struct RedCtl {
 };
 
 // Block sums of v[NR] -> partials[r * gridDim.x + blockIdx.x] (thread 0 writes).
+// blockDim.x must be a multiple of 64 and at most kBlock.
 template <int NR>
 __device__ __forceinline__ void block_partials(const double (&v)[NR], double* partials) {
   __shared__ double sm[NR][kBlock / 64];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     const double t = wave_sum(v[r]);
@@ -71,8 +73,7 @@ __device__ __forceinline__ void block_partials(const double (&v)[NR], double* pa
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       double t = 0.0;
-#pragma unroll
-      for (int i = 0; i < kBlock / 64; ++i) t += sm[r][i];
+      for (int i = 0; i < nw; ++i) t += sm[r][i];
       partials[(int64_t)r * gridDim.x + blockIdx.x] = t;
     }
   }

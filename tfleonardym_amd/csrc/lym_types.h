@@ -273,6 +273,21 @@ struct Lat {
   int has_bc;           // bc factor != 1
 };
 
+// Block schedule of the Wilson kernels.  tiled = 1 (4D lattices that allow it):
+// a block covers k whole lines of the fastest direction (T = L_3 sites each,
+// so loads stay coalesced), XCD x owns the slab x0 in [x*slab, (x+1)*slab),
+// and inside an XCD the blocks run with x0 fastest, then groups of k x2
+// values, then x1 -- every neighbour a block reads was touched by a block a
+// few positions earlier on the same XCD, i.e. is still in that XCD's L2.
+// tiled = 0: consecutive sites, block ranges contiguous per XCD.
+struct SiteMap {
+  int tiled;
+  int T;     // L_3
+  int k;     // lines per block
+  int slab;  // x0 values per XCD
+  int ng;    // L_2 / k
+};
+
 template <int D>
 __host__ __device__ __forceinline__ void coords(const Lat& lat, int64_t s, int (&x)[D]) {
   uint32_t r = (uint32_t)s;

@@ -541,7 +541,7 @@ static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Fi
   LYM_HIP(hipMemcpyAsync(P.xs_dev, hx, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
   LYM_HIP(hipMemcpyAsync(P.ps_dev, hp, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
   const int64_t n = field_elems(d);
-  const int gw = grid_for(d.lat.V);
+  const int gw = d.wblocks;  // Wilson grid: number of partials its reductions write
   const int gr = red_grid_flat(n);
   const int gf = flat_grid(n);
   if (prec == P64)
@@ -610,7 +610,7 @@ static int cgm_fast(const OpDesc& d, const Parts& P, Field b, void* const* xs, i
   LYM_HIP(hipMemcpyAsync(P.xs_dev, hx, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
   LYM_HIP(hipMemcpyAsync(P.ps_dev, hp, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
   const int64_t n = field_elems(d);
-  const int gw = grid_for(d.lat.V);
+  const int gw = d.wblocks;  // Wilson grid: number of partials its reductions write
   const int gr = red_grid_flat(n);
   const int gf = flat_grid(n);
   const int nslot = kFastBlock + 1;
@@ -819,7 +819,7 @@ int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const
   LYM_HIP(hipMemcpyAsync(P.st, &h, sizeof(h), hipMemcpyHostToDevice, st));
   LYM_HIP(hipMemcpyAsync(P.xs_dev, hx, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
   const int64_t n = field_elems(d);
-  const int gw = grid_for(d.lat.V);
+  const int gw = d.wblocks;  // Wilson grid: number of partials its reductions write
   const int gr = red_grid_flat(n);
   const int gf = flat_grid(n);
   hipLaunchKernelGGL(k_mr_init, dim3(gf), dim3(kBlock), 0, st, n, (const double2*)b, (double2*)P.r64.data, P.xs_dev,

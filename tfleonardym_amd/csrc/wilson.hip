@@ -221,19 +221,21 @@ struct EpiMrK2 {
 // The kernel
 // ---------------------------------------------------------------------------
 template <int D, int NC, bool ADJ, bool PROJ, int P, class Epi>
-__global__ __launch_bounds__(kBlock) void k_wilson(Lat lat, SpinorView<P> in, LinkView<typename Store<P>::R> lk,
+__global__ __launch_bounds__(kBlock) void k_wilson(Lat lat, SiteMap map, SpinorView<P> in,
+                                                   LinkView<typename Store<P>::R> lk,
                                                    LinkView<typename Store<P>::R> lkr, Epi epi, RedCtl rc) {
   using R = typename Store<P>::R;
   constexpr int NCOMP = Gam<D>::NS * (ADJ ? NC * NC : NC);
   if (!epi.active()) return;
-  const int64_t s = xcd_block() * kBlock + threadIdx.x;
   constexpr int NR = Epi::NRED > 0 ? Epi::NRED : 1;
   double red[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) red[r] = 0.0;
-  if (s < lat.V) {
+  int64_t s;
+  int x[D];
+  if (site_of_thread<D>(lat, map, s, x)) {
     cpx<R> hop[NCOMP];
-    wilson_hop<D, NC, ADJ, PROJ, R>(lat, s, in, lk, lkr, hop);
+    wilson_hop_x<D, NC, ADJ, PROJ, R>(lat, s, x, in, lk, lkr, hop);
     epi.apply(lat, s, in, hop, red);
   }
   if constexpr (Epi::NRED > 0) block_partials<NR>(red, rc.partials);
@@ -258,14 +260,14 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
   LinkView<R> lkr{(const cpx<R>*)lr, V};
   SpinorView<P> sv{(typename Store<P>::elem*)in.data, in.norm, V};
   Epi epi(a, V);
-  const int grid = grid_for(V);
+  const dim3 grid(d.wblocks), block(d.wthreads);
   if (proj) {
-    hipLaunchKernelGGL((k_wilson<D, NC, ADJ, true, P, Epi>), dim3(grid), dim3(kBlock), 0, stream, d.lat, sv, lk, lkr,
-                       epi, rc);
+    hipLaunchKernelGGL((k_wilson<D, NC, ADJ, true, P, Epi>), grid, block, 0, stream, d.lat, d.map, sv, lk, lkr, epi,
+                       rc);
   } else {
     if constexpr (P == P64 && std::is_same<Epi, EpiStore<P, NCOMP>>::value) {
-      hipLaunchKernelGGL((k_wilson<D, NC, ADJ, false, P, Epi>), dim3(grid), dim3(kBlock), 0, stream, d.lat, sv, lk,
-                         lkr, epi, rc);
+      hipLaunchKernelGGL((k_wilson<D, NC, ADJ, false, P, Epi>), grid, block, 0, stream, d.lat, d.map, sv, lk, lkr,
+                         epi, rc);
     } else {
       set_error("simple (full-projector) mode is only built for the plain complex128 apply");
       return LYM_ERR_UNSUPPORTED;

@@ -12,6 +12,7 @@
 //   lower rows                     :  psi_{Ns/2+perm_r} part = -/+ i^-k_r h_r
 // so the projection and the reconstruction are exact sign/swap operations.
 #pragma once
+#include "lym_reduce.h"
 #include "lym_types.h"
 
 namespace lym {
@@ -134,17 +135,15 @@ __host__ __device__ __forceinline__ void transport(const cpx<R> (&u)[NC][NC], co
 //   PROJ = false: full projector on all Ns rows (dirac.py:503-523)
 // SV: spinor view with load<NS*NCC>(site, v); links/links_r: LinkView<R>.
 template <int D, int NC, bool ADJ, bool PROJ, typename R, typename SV>
-__host__ __device__ __forceinline__ void wilson_hop(const Lat& lat, int64_t s, const SV& psi,
-                                                    const LinkView<R>& links, const LinkView<R>& links_r,
-                                                    cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
+__host__ __device__ __forceinline__ void wilson_hop_x(const Lat& lat, int64_t s, const int (&x)[D], const SV& psi,
+                                                      const LinkView<R>& links, const LinkView<R>& links_r,
+                                                      cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
   constexpr int NS = Gam<D>::NS;
   constexpr int H = NS / 2;
   constexpr int NCC = ADJ ? NC * NC : NC;
   constexpr int NCOMP = NS * NCC;
 #pragma unroll
   for (int c = 0; c < NCOMP; ++c) acc[c] = czero<R>();
-  int x[D];
-  coords<D>(lat, s, x);
 
 #pragma unroll
   for (int mu = 0; mu < D; ++mu) {
@@ -228,6 +227,41 @@ __host__ __device__ __forceinline__ void wilson_hop(const Lat& lat, int64_t s, c
       }
     }
   }
+}
+
+// hop of site s, coordinates computed from the site index
+template <int D, int NC, bool ADJ, bool PROJ, typename R, typename SV>
+__host__ __device__ __forceinline__ void wilson_hop(const Lat& lat, int64_t s, const SV& psi,
+                                                    const LinkView<R>& links, const LinkView<R>& links_r,
+                                                    cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
+  int x[D];
+  coords<D>(lat, s, x);
+  wilson_hop_x<D, NC, ADJ, PROJ, R, SV>(lat, s, x, psi, links, links_r, acc);
+}
+
+// Site and coordinates of this thread under the block schedule `m`; false if
+// the thread has no site (tail of a linear schedule).
+template <int D>
+__device__ __forceinline__ bool site_of_thread(const Lat& lat, const SiteMap& m, int64_t& s, int (&x)[D]) {
+  if constexpr (D == 4) {
+    if (m.tiled) {
+      const int xcd = blockIdx.x % kNumXcd;
+      const int j = blockIdx.x / kNumXcd;
+      const int x0l = j % m.slab;
+      const int r = j / m.slab;
+      const int g = r % m.ng;
+      x[0] = xcd * m.slab + x0l;
+      x[1] = r / m.ng;
+      x[2] = g * m.k + (int)threadIdx.x / m.T;
+      x[3] = (int)threadIdx.x % m.T;
+      s = (((int64_t)x[0] * lat.L[1] + x[1]) * lat.L[2] + x[2]) * m.T + x[3];
+      return true;
+    }
+  }
+  s = xcd_block() * blockDim.x + threadIdx.x;
+  if (s >= lat.V) return false;
+  coords<D>(lat, s, x);
+  return true;
 }
 
 // res = g5^herm (center + kappa * hop)
