@@ -146,6 +146,19 @@ int lym_cgm(const lym_wilson* op, int prec, int nshift, const double* shifts, co
             double tol, int max_iter, void* workspace, size_t workspace_bytes, lym_solve_stats* stats,
             void* stream);
 
+/* Multishift CG for nrhs right-hand sides sharing one shift list:
+ * xs[r * nshift + s] = (Q^2 + shifts[s])^-1 bs[r].  LYM_PREC_F32 runs all RHS
+ * in the same launches (one Wilson launch covers the batch, links are read
+ * once per site block and reused from L2 by the other RHS); LYM_PREC_F64 is
+ * the plain FP64 CG-M, one RHS after the other.  stats: nrhs entries or NULL.
+ * Workspace: lym_cgm_batched_workspace_bytes.  Synchronous.  New entry point
+ * (no counterpart in the reference, which solves one source per call,
+ * e.g. fermion_measurements.py:57-68, 101-108). */
+size_t lym_cgm_batched_workspace_bytes(const lym_wilson* op, int prec, int nshift, int nrhs);
+int lym_cgm_batched(const lym_wilson* op, int prec, int nrhs, int nshift, const double* shifts, const void* const* bs,
+                    void* const* xs, double tol, int max_iter, void* workspace, size_t workspace_bytes,
+                    lym_solve_stats* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

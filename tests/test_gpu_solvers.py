@@ -56,6 +56,31 @@ def test_cgm_true_residual(mode, geometry, nc):
     assert max(solver.last_stats["true_rel_resid"]) <= 1e-10
 
 
+@pytest.mark.parametrize("mode", ["mixed", "double"])
+@pytest.mark.parametrize("geometry", [(4, 4, 4, 8), (8, 4, 4, 16)])
+def test_batched_cgm(mode, geometry):
+    """Several sources (Gaussian, Z2, point) in one batched solve: every
+    (source, shift) reaches 1e-10 (oracle-recomputed) and agrees with the
+    single-source solve."""
+    from tfleonardym_amd import dirac, dirac_solver
+    print(f"case batched mode={mode} geometry={geometry}", flush=True)
+    op, oop = setup(geometry, 3, 0.13)
+    srcs = [fields.gaussian_spinor(geometry, 4, 3, seed=40),
+            fields.z2_noise(geometry, 4, 3, seed=41),
+            fields.point_source(geometry, 4, 3, site=0, color=1, spin=2)]
+    shifts = [0.5, 0.05, 0.002]
+    s = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode=mode,
+                                       verbose=False)
+    sols = s.solve_batch(dirac.SquareOperator(op), srcs, shifts)
+    assert s.last_status == "converged", s.last_stats
+    for b, row in zip(srcs, sols):
+        single = s.solve(dirac.SquareOperator(op), b, shifts)
+        for sh, x, x1 in zip(shifts, row, single):
+            xn = x.cpu().numpy()
+            assert osv.true_residual(oop.apply_square, b, xn, sh) <= 1e-10
+            assert rel(xn, x1.cpu().numpy()) < 1e-8
+
+
 def test_c1_plain_fp64_cg():
     """The CPU-runnable workload: 2D 24x16 SU(2), Haar links, kappa=-0.15,
     periodic x / antiperiodic t, FP64 CG on D^dag D to 1e-10."""

@@ -87,6 +87,10 @@ SIGNATURES = {
                                  ctypes.POINTER(LymSolveStats), _vp]),
     "lym_cgm": (_i32, [ctypes.POINTER(LymWilson), _i32, _i32, ctypes.POINTER(_f64), _vp, ctypes.POINTER(_vp),
                        _f64, _i32, _vp, _sz, ctypes.POINTER(LymSolveStats), _vp]),
+    "lym_cgm_batched_workspace_bytes": (_sz, [ctypes.POINTER(LymWilson), _i32, _i32, _i32]),
+    "lym_cgm_batched": (_i32, [ctypes.POINTER(LymWilson), _i32, _i32, _i32, ctypes.POINTER(_f64),
+                               ctypes.POINTER(_vp), ctypes.POINTER(_vp), _f64, _i32, _vp, _sz,
+                               ctypes.POINTER(LymSolveStats), _vp]),
 }
 
 _lib = None

@@ -101,9 +101,12 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   return LYM_OK;
 }
 
-size_t solver_workspace_bytes(const OpDesc& d, int prec, int nshift);
+size_t solver_workspace_bytes(const OpDesc& d, int prec, int nshift, int nrhs);
 int solve_cgm(const OpDesc& d, int prec, int nshift, const double* shifts, const void* b, void* const* xs, double tol,
               int max_iter, void* ws, size_t ws_bytes, lym_solve_stats* stats, hipStream_t st);
+int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const double* shifts, const void* const* bs,
+                      void* const* xs, double tol, int max_iter, void* ws, size_t ws_bytes, lym_solve_stats* stats,
+                      hipStream_t st);
 int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const void* b, void* const* xs,
              int max_steps, double precision, double omega, void* ws, size_t ws_bytes, lym_solve_stats* stats,
              hipStream_t st);
@@ -233,7 +236,14 @@ size_t lym_solver_workspace_bytes(const lym_wilson* op, int prec, int nshift) {
   OpDesc d;
   if (make_desc(op, &d)) return 0;
   if (nshift < 1 || nshift > kMaxShifts) return 0;
-  return solver_workspace_bytes(d, prec, nshift);
+  return solver_workspace_bytes(d, prec, nshift, 1);
+}
+
+size_t lym_cgm_batched_workspace_bytes(const lym_wilson* op, int prec, int nshift, int nrhs) {
+  OpDesc d;
+  if (make_desc(op, &d)) return 0;
+  if (nshift < 1 || nshift > kMaxShifts || nrhs < 1) return 0;
+  return solver_workspace_bytes(d, prec, nshift, prec == LYM_PREC_F64 ? 1 : nrhs);
 }
 
 int lym_mr_multishift(const lym_wilson* op, int nshift, const double* shifts, int first, const void* b,
@@ -263,6 +273,21 @@ int lym_cgm(const lym_wilson* op, int prec, int nshift, const double* shifts, co
   if (stats) std::memset(stats, 0, sizeof(*stats));
   return solve_cgm(d, prec, nshift, shifts, b, xs, tol, max_iter, workspace, workspace_bytes, stats,
                    (hipStream_t)stream);
+}
+
+int lym_cgm_batched(const lym_wilson* op, int prec, int nrhs, int nshift, const double* shifts, const void* const* bs,
+                    void* const* xs, double tol, int max_iter, void* workspace, size_t workspace_bytes,
+                    lym_solve_stats* stats, void* stream) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (!bs || !xs || !shifts || !(tol > 0.0) || max_iter < 1 || nrhs < 1) {
+    set_error("lym_cgm_batched: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  if (stats) std::memset(stats, 0, sizeof(*stats) * nrhs);
+  return solve_cgm_batched(d, prec, nrhs, nshift, shifts, bs, xs, tol, max_iter, workspace, workspace_bytes, stats,
+                           (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -43,6 +43,10 @@ struct EpiArgs {
   Field ring;
   int64_t slot_elems = 0;
   int nslot = 0;
+  // batched launches: RHS i uses fields offset by i * rhs_stride elements,
+  // rings by i * ring_rhs_stride, and SolverState st + i
+  int64_t rhs_stride = 0;
+  int64_t ring_rhs_stride = 0;
 };
 
 // error reporting (thread local message)
@@ -68,7 +72,7 @@ enum EpiKind : int {
   EPI_CGM_K2F = 6,   // fast CG-M: ring[k+1] = ring[k] - alpha (Op in + sigma0 aux0); deferred finalizer
 };
 int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, const EpiArgs& a, RedCtl rc,
-                  hipStream_t stream);
+                  hipStream_t stream, int nrhs = 1);
 
 // ---- finalizer kernel (solvers.hip): one block reduces the partials of the
 // preceding kernel in a fixed order, then advances the solver state.
@@ -83,8 +87,9 @@ enum FinKind : int {
   FK_MR = 7,          // gate K1: MR alpha and shift coefficients
   FK_MR_RR = 8,       // gate K3: MR stop test
 };
+// nb finalizer blocks: block i handles state st + i, partials + i * nr * G, out + i * nr
 int launch_finalize(FinKind kind, SolverState* st, const double* partials, int G, int nr, double* out,
-                    hipStream_t stream);
+                    hipStream_t stream, int nb = 1);
 
 // ---- vector kernels (blas.hip / solvers.hip) ------------------------------
 int blas_dot(int prec, int64_t n, const void* x, const void* y, double* partial_out_dev, RedCtl rc,

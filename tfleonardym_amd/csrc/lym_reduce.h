@@ -55,10 +55,12 @@ struct RedCtl {
   unsigned* ticket;  // unused by the two-pass scheme (kept for layout)
 };
 
-// Block sums of v[NR] -> partials[r * gridDim.x + blockIdx.x] (thread 0 writes).
-// blockDim.x must be a multiple of 64 and at most kBlock.
+// Block sums of v[NR] -> partials[r * G + idx] (thread 0 writes); by default
+// G = gridDim.x, idx = blockIdx.x.  blockDim.x must be a multiple of 64, <= kBlock.
 template <int NR>
-__device__ __forceinline__ void block_partials(const double (&v)[NR], double* partials) {
+__device__ __forceinline__ void block_partials(const double (&v)[NR], double* partials, int idx = -1, int G = -1) {
+  if (idx < 0) idx = blockIdx.x;
+  if (G < 0) G = gridDim.x;
   __shared__ double sm[NR][kBlock / 64];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -74,7 +76,7 @@ __device__ __forceinline__ void block_partials(const double (&v)[NR], double* pa
     for (int r = 0; r < NR; ++r) {
       double t = 0.0;
       for (int i = 0; i < nw; ++i) t += sm[r][i];
-      partials[(int64_t)r * gridDim.x + blockIdx.x] = t;
+      partials[(int64_t)r * G + idx] = t;
     }
   }
 }

@@ -56,6 +56,7 @@ struct SolverState {
   int mblk;
   int blk_j;         // steps recorded in the current block
   int precision_phase;  // 0 = FP64 Krylov vectors, 1 = low precision
+  int switch_counted;   // this RHS has been counted in the batch's switch tally
   double switch_rr;  // absolute threshold on rr for want_switch
   int n_upd[kMaxShifts];
   double h_alpha[kMaxBlock][kMaxShifts];

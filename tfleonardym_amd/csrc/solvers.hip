@@ -62,8 +62,21 @@ __device__ __forceinline__ E from64(double2 d) {
 // ---------------------------------------------------------------------------
 // finalizer kernel
 // ---------------------------------------------------------------------------
+// Batch bookkeeping the host polls: RHS that finished, RHS ready for the
+// precision switch (wanting it or finished).
+struct BatchCtl {
+  int ndone;
+  int nswitch;
+  int nrhs;
+  int pad;
+};
+
+// block i of the grid finalizes RHS i: state st + i, partials + i * nr * G
 __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const double* partials, int G, int kind, int nr,
-                                                     double* out) {
+                                                     double* out, BatchCtl* ctl) {
+  if (st) st += blockIdx.x;
+  if (partials) partials += (int64_t)blockIdx.x * nr * G;
+  if (out) out += (int64_t)blockIdx.x * nr;
   bool run = true;
   if (st) {
     const int ph = st->phase;
@@ -113,24 +126,41 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       break;
     case FK_CGM_A: cgm_fin_a(st, sums[0], sums[1]); break;
     case FK_CGM_B: cgm_fin_b(st, sums[0]); break;
-    case FK_CGM_BF: cgm_fin_b_fast(st, sums[0]); break;
+    case FK_CGM_BF:
+      cgm_fin_b_fast(st, sums[0]);
+      if (ctl && st->want_switch && !st->switch_counted) {
+        st->switch_counted = 1;
+        atomicAdd(&ctl->nswitch, 1);
+      }
+      break;
     case FK_CGM_K3ADV: st->phase = st->final_k3 ? PH_DONE : PH_K1; break;
     case FK_FAST_K3ADV:
       st->blk_j = 0;
       for (int k = 0; k < st->nshift; ++k) st->n_upd[k] = 0;
       st->phase = st->final_k3 ? PH_DONE : PH_KP;
+      if (ctl && st->phase == PH_DONE) {
+        atomicAdd(&ctl->ndone, 1);
+        if (!st->switch_counted) {
+          st->switch_counted = 1;
+          atomicAdd(&ctl->nswitch, 1);
+        }
+      }
       break;
     case FK_MR: mr_fin(st, sums[0], sums[1], sums[2]); break;
     case FK_MR_RR: mr_fin_rr(st, sums[0]); break;
   }
 }
 
-int launch_finalize(FinKind kind, SolverState* st, const double* partials, int G, int nr, double* out,
-                    hipStream_t stream) {
-  // partials rows beyond nr are read (as 4 rows) but ignored; keep G rows valid
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, stream, st, partials, G, (int)kind, nr, out);
+static int launch_fin(FinKind kind, SolverState* st, const double* partials, int G, int nr, double* out,
+                      hipStream_t stream, int nb, BatchCtl* ctl) {
+  hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(kBlock), 0, stream, st, partials, G, (int)kind, nr, out, ctl);
   LYM_HIP(hipGetLastError());
   return LYM_OK;
+}
+
+int launch_finalize(FinKind kind, SolverState* st, const double* partials, int G, int nr, double* out,
+                    hipStream_t stream, int nb) {
+  return launch_fin(kind, st, partials, G, nr, out, stream, nb, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -183,10 +213,18 @@ __global__ __launch_bounds__(kBlock) void k_cgm_k3(const SolverState* st, int64_
   }
 }
 
-// fast CG-M init: ring slot 0 = b, live p = 0, x_s = 0, p_s = b (FP64); partial |b|^2
-__global__ __launch_bounds__(kBlock) void k_fast_init(int64_t n, const double2* __restrict__ b, double2* __restrict__ r0,
-                                                      double2* __restrict__ plive, const Field* xs, const Field* ps,
-                                                      int nshift, double* partials) {
+// fast CG-M init (grid.y = RHS): ring slot 0 = b, live p = 0, x_s = 0,
+// p_s = b (FP64); partial |b|^2
+__global__ __launch_bounds__(kBlock) void k_fast_init(int64_t n, const Field* bs, double2* __restrict__ ring64,
+                                                      int64_t ring_stride, double2* __restrict__ plive64,
+                                                      const Field* xs, const Field* ps, int nshift, double* partials) {
+  const int rhs = blockIdx.y;
+  const double2* __restrict__ b = (const double2*)bs[rhs].data;
+  double2* __restrict__ r0 = ring64 + rhs * ring_stride;
+  double2* __restrict__ plive = plive64 + rhs * n;
+  xs += rhs * nshift;
+  ps += rhs * nshift;
+  partials += (int64_t)rhs * gridDim.x;
   double red[1] = {0.0};
   LYM_GRID_STRIDE(i, n) {
     const double2 v = b[i];
@@ -205,10 +243,14 @@ __global__ __launch_bounds__(kBlock) void k_fast_init(int64_t n, const double2* 
 // p = r^k + beta_{k-1} p   (r^k in ring slot k % nslot)
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_fast_kp(const SolverState* st, int64_t n, const Elem<P>* __restrict__ ring,
-                                                    int nslot, Elem<P>* __restrict__ p) {
+                                                    int64_t ring_stride, int nslot, Elem<P>* __restrict__ p) {
   using E = Elem<P>;
   using R = typename Store<P>::R;
+  const int rhs = blockIdx.y;
+  st += rhs;
   if (st->phase != PH_KP) return;
+  ring += rhs * ring_stride;
+  p += rhs * n;
   const E* r = ring + (int64_t)(st->iter % nslot) * n;
   const R b = (R)st->beta_old;
   LYM_GRID_STRIDE(i, n) {
@@ -225,8 +267,14 @@ __global__ __launch_bounds__(kBlock) void k_fast_kp(const SolverState* st, int64
 // (x_s, p_s in FP64; each residual element read once per block).
 template <int PR, int MB>
 __global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int64_t n, const Elem<PR>* __restrict__ ring,
-                                                    int nslot, const Field* xs, const Field* ps, int nshift) {
+                                                    int64_t ring_stride, int nslot, const Field* xs, const Field* ps,
+                                                    int nshift) {
+  const int rhs = blockIdx.y;
+  st += rhs;
   if (st->phase != PH_K3) return;
+  ring += rhs * ring_stride;
+  xs += rhs * nshift;
+  ps += rhs * nshift;
   const int nb = st->blk_j;
   const int k0 = st->iter - nb;
   LYM_GRID_STRIDE(i, n) {
@@ -262,8 +310,14 @@ __global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int64
 // residual (ring slot iter % nslot) and of the live direction
 template <int PL>
 __global__ __launch_bounds__(kBlock) void k_fast_switch(SolverState* st, int64_t n, const double2* __restrict__ ring64,
-                                                        Elem<PL>* __restrict__ ringlo, int nslot,
+                                                        Elem<PL>* __restrict__ ringlo, int64_t ring_stride, int nslot,
                                                         const double2* __restrict__ p64, Elem<PL>* __restrict__ plo) {
+  const int rhs = blockIdx.y;
+  st += rhs;
+  ring64 += rhs * ring_stride;
+  ringlo += rhs * ring_stride;
+  p64 += rhs * n;
+  plo += rhs * n;
   const int64_t off = (int64_t)(st->iter % nslot) * n;
   LYM_GRID_STRIDE(i, n) {
     ringlo[off + i] = from64<Elem<PL>>(ring64[off + i]);
@@ -366,31 +420,38 @@ static Field take_field(Carver& c, const OpDesc& d, int prec) {
 constexpr int kFastBlock = 8;  // iterations per deferred shift-update block
 
 struct Parts {
-  SolverState* st;
-  double* partials;
+  SolverState* st;  // nrhs states
+  BatchCtl* ctl;
+  double* partials;  // nrhs x kMaxRed x max_red_blocks
   double* results;
-  Field* xs_dev;
-  Field* ps_dev;
-  Field r64, t64, u64, v64;
-  Field b_lo, r_lo, t_lo;  // low-precision single-shift defect correction
+  Field* xs_dev;  // nrhs x nshift
+  Field* ps_dev;  // nrhs x nshift
+  Field* bs_dev;  // nrhs sources
+  int nrhs = 1;
+  Field r64, t64, u64, v64;  // single-field FP64 temporaries (true residuals, refinement)
+  Field b_lo, r_lo, t_lo;    // low-precision single-shift defect correction
   Field x_lo[kMaxShifts];
-  Field p[kMaxShifts];     // search directions at the solve precision
-  // fast CG-M (mixed): residual rings, live base direction, FP64 shift directions
-  Field ring64, ring32;    // kFastBlock + 1 contiguous slots each
-  Field plive64, plive32;
-  Field ps64[kMaxShifts];
+  Field p[kMaxShifts];  // plain CG-M search directions
+  // fast CG-M (mixed), every field batched over nrhs RHS:
+  Field ring64, ring32;     // nrhs x (kFastBlock + 1) slots
+  Field plive64, plive32;   // live base direction
+  Field tb64, tb32;         // t = Q p
+  Field ps64;               // nrhs x nshift FP64 shift directions
 };
 
 // prec: P64 = plain FP64 solves (and MR); otherwise the fast mixed solver
-static size_t carve(const OpDesc& d, int prec, int nshift, char* base, Parts* pt) {
+static size_t carve(const OpDesc& d, int prec, int nshift, int nrhs, char* base, Parts* pt) {
   Carver c{base};
   Parts tmp;
   Parts& P = pt ? *pt : tmp;
-  P.st = (SolverState*)c.take(sizeof(SolverState));
-  P.partials = (double*)c.take(sizeof(double) * kMaxRed * (size_t)max_red_blocks(d.lat.V));
+  P.nrhs = nrhs;
+  P.st = (SolverState*)c.take(sizeof(SolverState) * nrhs);
+  P.ctl = (BatchCtl*)c.take(sizeof(BatchCtl));
+  P.partials = (double*)c.take(sizeof(double) * kMaxRed * (size_t)max_red_blocks(d.lat.V) * nrhs);
   P.results = (double*)c.take(sizeof(double) * 16);
-  P.xs_dev = (Field*)c.take(sizeof(Field) * kMaxShifts);
-  P.ps_dev = (Field*)c.take(sizeof(Field) * kMaxShifts);
+  P.xs_dev = (Field*)c.take(sizeof(Field) * kMaxShifts * nrhs);
+  P.ps_dev = (Field*)c.take(sizeof(Field) * kMaxShifts * nrhs);
+  P.bs_dev = (Field*)c.take(sizeof(Field) * nrhs);
   P.r64 = take_field(c, d, P64);
   P.t64 = take_field(c, d, P64);
   P.u64 = take_field(c, d, P64);
@@ -403,16 +464,20 @@ static size_t carve(const OpDesc& d, int prec, int nshift, char* base, Parts* pt
     P.t_lo = take_field(c, d, P32);
     P.x_lo[0] = take_field(c, d, P32);
     P.p[0] = take_field(c, d, P32);
-    P.ring64.data = c.take(field_bytes(d, P64) * (kFastBlock + 1));
-    P.ring32.data = c.take(field_bytes(d, P32) * (kFastBlock + 1));
-    P.plive64 = take_field(c, d, P64);
-    P.plive32 = take_field(c, d, P32);
-    for (int k = 0; k < nshift; ++k) P.ps64[k] = take_field(c, d, P64);
+    P.ring64.data = c.take(field_bytes(d, P64) * (kFastBlock + 1) * nrhs);
+    P.ring32.data = c.take(field_bytes(d, P32) * (kFastBlock + 1) * nrhs);
+    P.plive64.data = c.take(field_bytes(d, P64) * nrhs);
+    P.plive32.data = c.take(field_bytes(d, P32) * nrhs);
+    P.tb64.data = c.take(field_bytes(d, P64) * nrhs);
+    P.tb32.data = c.take(field_bytes(d, P32) * nrhs);
+    P.ps64.data = c.take(field_bytes(d, P64) * nshift * nrhs);
   }
   return c.off + 256;
 }
 
-size_t solver_workspace_bytes(const OpDesc& d, int prec, int nshift) { return carve(d, prec, nshift, nullptr, nullptr); }
+size_t solver_workspace_bytes(const OpDesc& d, int prec, int nshift, int nrhs) {
+  return carve(d, prec, nshift, nrhs, nullptr, nullptr);
+}
 
 static RedCtl redctl(const Parts& P) { return RedCtl{P.partials, nullptr}; }
 
@@ -588,56 +653,67 @@ static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Fi
   return LYM_OK;
 }
 
-// Fast multishift CG: blocks of kFastBlock iterations
+// Fast multishift CG over a batch of nrhs right-hand sides (one shift list).
+// Blocks of kFastBlock iterations:
 //   [ p = r + beta p ; t = Q p (|t|^2,|p|^2) ; r' = r - alpha (Q t + s0 p) (|r'|^2) ] x m
 //   then one deferred update of every shift's x_s, p_s (FP64) from the ring.
-// Krylov vectors are FP64 until want_switch (base rr <= switch_tol^2 |b|^2),
-// then `lowprec`.  xs are the caller's FP64 solutions.
-static int cgm_fast(const OpDesc& d, const Parts& P, Field b, void* const* xs, int nshift, const double* shifts,
-                    const double* tol_rel, int max_iter, double switch_tol, int lowprec, hipStream_t st,
-                    SolverState* out_state) {
+// Krylov vectors are FP64 until every RHS has its base residual below
+// switch_tol |b| (or has finished), then `lowprec`.  xs[r * nshift + s] are the
+// caller's FP64 solutions.
+static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void* const* xs, int nrhs, int nshift,
+                    const double* shifts, const double* tol_rel, int max_iter, double switch_tol, int lowprec,
+                    hipStream_t st, std::vector<SolverState>& out_states) {
   SolverState h;
   init_cgm_state(h, nshift, shifts, tol_rel, max_iter);
   h.phase = PH_KP;
   h.mblk = kFastBlock;
   h.switch_rr = switch_tol * switch_tol;
-  Field hx[kMaxShifts], hp[kMaxShifts];
-  for (int k = 0; k < nshift; ++k) {
-    hx[k].data = xs[k];
-    hp[k] = P.ps64[k];
-  }
-  LYM_HIP(hipMemcpyAsync(P.st, &h, sizeof(h), hipMemcpyHostToDevice, st));
-  LYM_HIP(hipMemcpyAsync(P.xs_dev, hx, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
-  LYM_HIP(hipMemcpyAsync(P.ps_dev, hp, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
+  std::vector<SolverState> hs(nrhs, h);
+  std::vector<Field> hx((size_t)nrhs * nshift), hp((size_t)nrhs * nshift), hb(nrhs);
   const int64_t n = field_elems(d);
-  const int gw = d.wblocks;  // Wilson grid: number of partials its reductions write
+  for (int r = 0; r < nrhs; ++r) {
+    hb[r].data = const_cast<void*>(bs[r]);
+    for (int k = 0; k < nshift; ++k) {
+      hx[(size_t)r * nshift + k].data = xs[(size_t)r * nshift + k];
+      hp[(size_t)r * nshift + k].data = (char*)P.ps64.data + ((int64_t)r * nshift + k) * n * 16;
+    }
+  }
+  BatchCtl ctl{0, 0, nrhs, 0};
+  LYM_HIP(hipMemcpyAsync(P.st, hs.data(), sizeof(SolverState) * nrhs, hipMemcpyHostToDevice, st));
+  LYM_HIP(hipMemcpyAsync(P.ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice, st));
+  LYM_HIP(hipMemcpyAsync(P.xs_dev, hx.data(), sizeof(Field) * hx.size(), hipMemcpyHostToDevice, st));
+  LYM_HIP(hipMemcpyAsync(P.ps_dev, hp.data(), sizeof(Field) * hp.size(), hipMemcpyHostToDevice, st));
+  LYM_HIP(hipMemcpyAsync(P.bs_dev, hb.data(), sizeof(Field) * nrhs, hipMemcpyHostToDevice, st));
+  const int gw = d.wblocks;
   const int gr = red_grid_flat(n);
   const int gf = flat_grid(n);
   const int nslot = kFastBlock + 1;
-  hipLaunchKernelGGL(k_fast_init, dim3(gr), dim3(kBlock), 0, st, n, (const double2*)b.data, (double2*)P.ring64.data,
-                     (double2*)P.plive64.data, P.xs_dev, P.ps_dev, nshift, P.partials);
+  const int64_t ring_stride = (int64_t)nslot * n;
+  hipLaunchKernelGGL(k_fast_init, dim3(gr, nrhs), dim3(kBlock), 0, st, n, P.bs_dev, (double2*)P.ring64.data,
+                     ring_stride, (double2*)P.plive64.data, P.xs_dev, P.ps_dev, nshift, P.partials);
   LYM_HIP(hipGetLastError());
-  int rc = launch_finalize(FK_INIT, P.st, P.partials, gr, 1, nullptr, st);
+  int rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
   if (rc) return rc;
 
   int prec = P64;
   auto chunk = [&]() -> int {
     const Field ring = prec == P64 ? P.ring64 : P.ring32;
     const Field pl = prec == P64 ? P.plive64 : P.plive32;
-    const Field t = prec == P64 ? P.t64 : P.t_lo;
+    const Field t = prec == P64 ? P.tb64 : P.tb32;
     for (int j = 0; j < kFastBlock; ++j) {
       if (prec == P64)
-        hipLaunchKernelGGL(k_fast_kp<P64>, dim3(gf), dim3(kBlock), 0, st, P.st, n, (const double2*)ring.data, nslot,
-                           (double2*)pl.data);
+        hipLaunchKernelGGL(k_fast_kp<P64>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const double2*)ring.data,
+                           ring_stride, nslot, (double2*)pl.data);
       else
-        hipLaunchKernelGGL(k_fast_kp<P32>, dim3(gf), dim3(kBlock), 0, st, P.st, n, (const float2*)ring.data, nslot,
-                           (float2*)pl.data);
+        hipLaunchKernelGGL(k_fast_kp<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const float2*)ring.data,
+                           ring_stride, nslot, (float2*)pl.data);
       LYM_HIP(hipGetLastError());
       EpiArgs a1;
       a1.out = t;
       a1.st = P.st;
-      int e = wilson_launch(EPI_CGM_K1, d, prec, true, pl, a1, redctl(P), st);
-      if (!e) e = launch_finalize(FK_CGM_A, P.st, P.partials, gw, 2, nullptr, st);
+      a1.rhs_stride = n;
+      int e = wilson_launch(EPI_CGM_K1, d, prec, true, pl, a1, redctl(P), st, nrhs);
+      if (!e) e = launch_fin(FK_CGM_A, P.st, P.partials, gw, 2, nullptr, st, nrhs, P.ctl);
       if (e) return e;
       EpiArgs a2;
       a2.aux0 = pl;
@@ -645,31 +721,58 @@ static int cgm_fast(const OpDesc& d, const Parts& P, Field b, void* const* xs, i
       a2.ring = ring;
       a2.slot_elems = n;
       a2.nslot = nslot;
-      e = wilson_launch(EPI_CGM_K2F, d, prec, true, t, a2, redctl(P), st);
-      if (!e) e = launch_finalize(FK_CGM_BF, P.st, P.partials, gw, 1, nullptr, st);
+      a2.rhs_stride = n;
+      a2.ring_rhs_stride = ring_stride;
+      e = wilson_launch(EPI_CGM_K2F, d, prec, true, t, a2, redctl(P), st, nrhs);
+      if (!e) e = launch_fin(FK_CGM_BF, P.st, P.partials, gw, 1, nullptr, st, nrhs, P.ctl);
       if (e) return e;
     }
     if (prec == P64)
-      hipLaunchKernelGGL((k_fast_k3<P64, kFastBlock>), dim3(gf), dim3(kBlock), 0, st, P.st, n,
-                         (const double2*)ring.data, nslot, P.xs_dev, P.ps_dev, nshift);
+      hipLaunchKernelGGL((k_fast_k3<P64, kFastBlock>), dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
+                         (const double2*)ring.data, ring_stride, nslot, P.xs_dev, P.ps_dev, nshift);
     else
-      hipLaunchKernelGGL((k_fast_k3<P32, kFastBlock>), dim3(gf), dim3(kBlock), 0, st, P.st, n,
-                         (const float2*)ring.data, nslot, P.xs_dev, P.ps_dev, nshift);
+      hipLaunchKernelGGL((k_fast_k3<P32, kFastBlock>), dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
+                         (const float2*)ring.data, ring_stride, nslot, P.xs_dev, P.ps_dev, nshift);
     LYM_HIP(hipGetLastError());
-    return launch_finalize(FK_FAST_K3ADV, P.st, nullptr, 0, 0, nullptr, st);
+    return launch_fin(FK_FAST_K3ADV, P.st, nullptr, 0, 0, nullptr, st, nrhs, P.ctl);
   };
-  auto on_poll = [&](const int* s4) -> int {
-    if (s4[2] && prec == P64 && lowprec == P32) {
-      hipLaunchKernelGGL(k_fast_switch<P32>, dim3(gf), dim3(kBlock), 0, st, P.st, n, (const double2*)P.ring64.data,
-                         (float2*)P.ring32.data, nslot, (const double2*)P.plive64.data, (float2*)P.plive32.data);
+  // poll the batch control block: (ndone, nswitch, nrhs, -)
+  int rce = ensure_sync();
+  if (rce) return rce;
+  const int64_t max_chunks = (int64_t)max_iter / kFastBlock + 8;
+  int64_t issued = 0, polled = 0;
+  auto issue = [&](int slot) -> int {
+    const int e = chunk();
+    if (e) return e;
+    LYM_HIP(hipMemcpyAsync(&g_sync.pinned[4 * slot], P.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    LYM_HIP(hipEventRecord(g_sync.ev[slot], st));
+    ++issued;
+    return LYM_OK;
+  };
+  if ((rc = issue(0))) return rc;
+  if ((rc = issue(1))) return rc;
+  while (true) {
+    const int slot = (int)(polled & 1);
+    LYM_HIP(hipEventSynchronize(g_sync.ev[slot]));
+    ++polled;
+    const int* s4 = &g_sync.pinned[4 * slot];
+    if (s4[0] >= nrhs) break;
+    if (issued > max_chunks) {
+      set_error("fast CG-M did not reach its stop state (internal error)");
+      return LYM_ERR_HIP;
+    }
+    if (s4[1] >= nrhs && prec == P64 && lowprec == P32) {
+      hipLaunchKernelGGL(k_fast_switch<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
+                         (const double2*)P.ring64.data, (float2*)P.ring32.data, ring_stride, nslot,
+                         (const double2*)P.plive64.data, (float2*)P.plive32.data);
       LYM_HIP(hipGetLastError());
       prec = P32;
     }
-    return (int)LYM_OK;
-  };
-  rc = run_chunks(P.st, (int64_t)max_iter / kFastBlock + 8, st, chunk, on_poll);
-  if (rc) return rc;
-  LYM_HIP(hipMemcpyAsync(out_state, P.st, sizeof(SolverState), hipMemcpyDeviceToHost, st));
+    if ((rc = issue(slot))) return rc;
+  }
+  LYM_HIP(hipStreamSynchronize(st));
+  out_states.resize(nrhs);
+  LYM_HIP(hipMemcpyAsync(out_states.data(), P.st, sizeof(SolverState) * nrhs, hipMemcpyDeviceToHost, st));
   LYM_HIP(hipStreamSynchronize(st));
   return LYM_OK;
 }
@@ -683,8 +786,69 @@ static void fill_stats(lym_solve_stats* s, const SolverState& h, int nshift) {
   for (int k = 0; k < nshift && k < 64; ++k) s->conv_iter[k] = h.conv_iter[k];
 }
 
-int solve_cgm(const OpDesc& d, int prec, int nshift, const double* shifts, const void* b, void* const* xs, double tol,
-              int max_iter, void* ws, size_t ws_bytes, lym_solve_stats* stats, hipStream_t st) {
+// FP64 true residual of every (rhs, shift) and defect correction where one
+// misses the target.  xs[r * nshift + k]; rel[r * nshift + k] receives the
+// final true relative residuals; returns the number of correction cycles.
+static int verify_and_refine(const OpDesc& d, const Parts& P, int prec, int nrhs, int nshift, const double* shifts,
+                             const void* const* bs, void* const* xs, double tol, int max_iter, bool breakdown,
+                             hipStream_t st, std::vector<double>& rel, int* restarts) {
+  const int64_t n = field_elems(d);
+  const int max_refine = 12;
+  rel.assign((size_t)nrhs * nshift, 0.0);
+  for (int r = 0; r < nrhs; ++r) {
+    Field bf;
+    bf.data = const_cast<void*>(bs[r]);
+    for (int k = 0; k < nshift; ++k) {
+      void* xk_ptr = xs[(size_t)r * nshift + k];
+      Field xk;
+      xk.data = xk_ptr;
+      double r2 = 0;
+      int rc = true_residual(d, P, bf, xk, shifts[k], P.r64, &r2, st);
+      if (rc) return rc;
+      int cycles = 0;
+      while (std::sqrt(r2) > tol && cycles < max_refine && !breakdown) {
+        const double relnow = std::sqrt(r2);
+        double tol_in = (prec == P64) ? 0.5 * tol / relnow : std::max(0.5 * tol / relnow, 1e-5);
+        if (tol_in > 0.1) tol_in = 0.1;
+        SolverState hi;
+        double sh = shifts[k];
+        if (prec == P64) {
+          // FP64 correction e (u64) from the residual copied to v64
+          rc = blas_convert(P64, P.r64.data, P64, P.v64.data, n, st);
+          if (rc) return rc;
+          Field xe[1] = {P.u64};
+          rc = cgm_core(d, P64, P, P.v64, xe, P.r64, P.t64, 1, &sh, &tol_in, max_iter, st, &hi);
+          if (rc) return rc;
+          rc = blas_axpby(P64, n, 1.0, 0.0, P.u64.data, 1.0, 0.0, xk_ptr, st);
+          if (rc) return rc;
+        } else {
+          rc = blas_convert(P64, P.r64.data, P32, P.b_lo.data, n, st);
+          if (rc) return rc;
+          Field xe[1] = {P.x_lo[0]};
+          rc = cgm_core(d, P32, P, P.b_lo, xe, P.r_lo, P.t_lo, 1, &sh, &tol_in, max_iter, st, &hi);
+          if (rc) return rc;
+          hipLaunchKernelGGL(k_add32to64, dim3(flat_grid(n)), dim3(kBlock), 0, st, n, (const float2*)P.x_lo[0].data,
+                             (double2*)xk_ptr);
+          LYM_HIP(hipGetLastError());
+        }
+        ++*restarts;
+        ++cycles;
+        rc = true_residual(d, P, bf, xk, shifts[k], P.r64, &r2, st);
+        if (rc) return rc;
+      }
+      rel[(size_t)r * nshift + k] = std::sqrt(r2);
+    }
+  }
+  return LYM_OK;
+}
+
+// Multishift CG for nrhs right-hand sides sharing one shift list.
+//   prec = P64: the plain FP64 CG-M, one RHS after the other.
+//   prec = P32: the fast mixed solver, all RHS in the same launches.
+// stats: nrhs entries (or NULL).
+int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const double* shifts, const void* const* bs,
+                      void* const* xs, double tol, int max_iter, void* ws, size_t ws_bytes, lym_solve_stats* stats,
+                      hipStream_t st) {
   const auto t0 = std::chrono::steady_clock::now();
   if (!d.herm) {
     set_error("CG needs a Hermitian operator: build the Wilson operator with hermitian=True");
@@ -692,6 +856,10 @@ int solve_cgm(const OpDesc& d, int prec, int nshift, const double* shifts, const
   }
   if (nshift < 1 || nshift > kMaxShifts) {
     set_error("number of shifts must be in [1, 64]");
+    return LYM_ERR_ARG;
+  }
+  if (nrhs < 1) {
+    set_error("number of right-hand sides must be >= 1");
     return LYM_ERR_ARG;
   }
   for (int k = 0; k < nshift; ++k)
@@ -703,86 +871,66 @@ int solve_cgm(const OpDesc& d, int prec, int nshift, const double* shifts, const
     set_error("half-precision CG-M is not built yet");
     return LYM_ERR_UNSUPPORTED;
   }
-  const size_t need = solver_workspace_bytes(d, prec, nshift);
+  const int nbatch = prec == P64 ? 1 : nrhs;  // plain FP64 runs one RHS at a time
+  const size_t need = solver_workspace_bytes(d, prec, nshift, nbatch);
   if (!ws || ws_bytes < need) {
     set_error("solver workspace too small: need " + std::to_string(need) + " bytes");
     return LYM_ERR_ARG;
   }
   Parts P;
   char* base = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
-  carve(d, prec, nshift, base, &P);
-  const int64_t n = field_elems(d);
-  Field bf;
-  bf.data = const_cast<void*>(b);
+  carve(d, prec, nshift, nbatch, base, &P);
   // iterate slightly below the target so the recomputed true residual meets it
   std::vector<double> tol_rel(nshift, 0.5 * tol);
-  SolverState h;
+  std::vector<SolverState> hs;
   int rc = 0;
-  int restarts = 0;
-  std::vector<double> rel(nshift, 0.0);
   if (prec == P64) {
-    Field xf[kMaxShifts];
-    for (int k = 0; k < nshift; ++k) xf[k].data = xs[k];
-    rc = cgm_core(d, P64, P, bf, xf, P.r64, P.t64, nshift, shifts, tol_rel.data(), max_iter, st, &h);
-  } else {
-    // mixed: FP64 Krylov vectors until the base residual is below 1e-4 of |b|,
-    // FP32 after; shift solutions and directions accumulate in FP64 through
-    // deferred block updates.  FP64 true residuals are checked below.
-    rc = cgm_fast(d, P, bf, xs, nshift, shifts, tol_rel.data(), max_iter, 1e-4, P32, st, &h);
-  }
-  if (rc) return rc;
-  // FP64 true residuals, and defect correction where a shift misses the target
-  const int max_refine = 12;
-  for (int k = 0; k < nshift; ++k) {
-    Field xk;
-    xk.data = xs[k];
-    double r2 = 0;
-    rc = true_residual(d, P, bf, xk, shifts[k], P.r64, &r2, st);
-    if (rc) return rc;
-    int cycles = 0;
-    while (std::sqrt(r2) > tol && cycles < max_refine && h.status != ST_BREAKDOWN) {
-      const double relnow = std::sqrt(r2);
-      double tol_in = (prec == P64) ? 0.5 * tol / relnow : std::max(0.5 * tol / relnow, 1e-5);
-      if (tol_in > 0.1) tol_in = 0.1;
-      SolverState hi;
-      double sh = shifts[k];
-      if (prec == P64) {
-        // FP64 correction e (u64) from the residual copied to v64
-        rc = blas_convert(P64, P.r64.data, P64, P.v64.data, n, st);
-        if (rc) return rc;
-        Field xe[1] = {P.u64};
-        rc = cgm_core(d, P64, P, P.v64, xe, P.r64, P.t64, 1, &sh, &tol_in, max_iter, st, &hi);
-        if (rc) return rc;
-        rc = blas_axpby(P64, n, 1.0, 0.0, P.u64.data, 1.0, 0.0, xs[k], st);
-        if (rc) return rc;
-      } else {
-        rc = blas_convert(P64, P.r64.data, P32, P.b_lo.data, n, st);
-        if (rc) return rc;
-        Field xe[1] = {P.x_lo[0]};
-        rc = cgm_core(d, P32, P, P.b_lo, xe, P.r_lo, P.t_lo, 1, &sh, &tol_in, max_iter, st, &hi);
-        if (rc) return rc;
-        hipLaunchKernelGGL(k_add32to64, dim3(flat_grid(n)), dim3(kBlock), 0, st, n, (const float2*)P.x_lo[0].data,
-                           (double2*)xs[k]);
-        LYM_HIP(hipGetLastError());
-      }
-      ++restarts;
-      ++cycles;
-      rc = true_residual(d, P, bf, xk, shifts[k], P.r64, &r2, st);
+    hs.resize(nrhs);
+    for (int r = 0; r < nrhs; ++r) {
+      Field bf;
+      bf.data = const_cast<void*>(bs[r]);
+      Field xf[kMaxShifts];
+      for (int k = 0; k < nshift; ++k) xf[k].data = xs[(size_t)r * nshift + k];
+      rc = cgm_core(d, P64, P, bf, xf, P.r64, P.t64, nshift, shifts, tol_rel.data(), max_iter, st, &hs[r]);
       if (rc) return rc;
     }
-    rel[k] = std::sqrt(r2);
+  } else {
+    // mixed: FP64 Krylov vectors until every base residual is below 1e-4 of
+    // its |b|, FP32 after; shift solutions and directions accumulate in FP64
+    // through deferred block updates.  FP64 true residuals are checked below.
+    rc = cgm_fast(d, P, bs, xs, nrhs, nshift, shifts, tol_rel.data(), max_iter, 1e-4, P32, st, hs);
+    if (rc) return rc;
   }
-  fill_stats(stats, h, nshift);
-  int status = h.status;
-  for (int k = 0; k < nshift; ++k)
-    if (rel[k] > tol && status == ST_CONVERGED) status = ST_MAXITER;
-  if (stats) {
-    stats->status = status;
-    stats->restarts = restarts;
-    for (int k = 0; k < nshift && k < 64; ++k) stats->true_rel_resid[k] = rel[k];
-    stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  bool breakdown = false;
+  for (auto& h : hs) breakdown |= (h.status == ST_BREAKDOWN);
+  std::vector<double> rel;
+  int restarts = 0;
+  rc = verify_and_refine(d, P, prec, nrhs, nshift, shifts, bs, xs, tol, max_iter, breakdown, st, rel, &restarts);
+  if (rc) return rc;
+  int worst = LYM_OK;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int r = 0; r < nrhs; ++r) {
+    int status = hs[r].status;
+    for (int k = 0; k < nshift; ++k)
+      if (rel[(size_t)r * nshift + k] > tol && status == ST_CONVERGED) status = ST_MAXITER;
+    if (stats) {
+      lym_solve_stats* s = &stats[r];
+      fill_stats(s, hs[r], nshift);
+      s->status = status;
+      s->restarts = restarts;
+      for (int k = 0; k < nshift && k < 64; ++k) s->true_rel_resid[k] = rel[(size_t)r * nshift + k];
+      s->seconds = secs;
+    }
+    const int code = status == ST_CONVERGED ? LYM_OK : (status == ST_BREAKDOWN ? LYM_BREAKDOWN : LYM_NOT_CONVERGED);
+    if (code > worst) worst = code;
   }
-  return status == ST_CONVERGED ? LYM_OK : (status == ST_BREAKDOWN ? LYM_BREAKDOWN : LYM_NOT_CONVERGED);
+  return worst;
+}
+
+int solve_cgm(const OpDesc& d, int prec, int nshift, const double* shifts, const void* b, void* const* xs, double tol,
+              int max_iter, void* ws, size_t ws_bytes, lym_solve_stats* stats, hipStream_t st) {
+  const void* bs[1] = {b};
+  return solve_cgm_batched(d, prec, 1, nshift, shifts, bs, xs, tol, max_iter, ws, ws_bytes, stats, st);
 }
 
 int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const void* b, void* const* xs,
@@ -793,14 +941,14 @@ int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const
     set_error("bad shift list for the multishift MR solver");
     return LYM_ERR_ARG;
   }
-  const size_t need = solver_workspace_bytes(d, P64, nshift);
+  const size_t need = solver_workspace_bytes(d, P64, nshift, 1);
   if (!ws || ws_bytes < need) {
     set_error("solver workspace too small: need " + std::to_string(need) + " bytes");
     return LYM_ERR_ARG;
   }
   Parts P;
   char* base = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
-  carve(d, P64, nshift, base, &P);
+  carve(d, P64, nshift, 1, base, &P);
   SolverState h;
   std::memset(&h, 0, sizeof(h));
   h.phase = PH_K1;

@@ -1,6 +1,8 @@
 // Wilson hopping kernels for gfx950: one site per lane, site-coalesced
-// component planes, XCD-contiguous block ranges, fused epilogues that store,
-// combine and reduce (FP64) in the same pass.
+// component planes, XCD-aware block schedule, fused epilogues that store,
+// combine and reduce (FP64) in the same pass.  A launch may cover a batch of
+// right-hand sides: fields of RHS i are offset by i * rhs_stride elements and
+// solver scalars by one SolverState per RHS.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -19,16 +21,37 @@ __device__ __forceinline__ double norm2(const cpx<R> (&v)[NCOMP]) {
   return a;
 }
 
+template <int P>
+__host__ __device__ __forceinline__ SpinorView<P> field_view(const Field& f, int64_t V) {
+  return SpinorView<P>{(typename Store<P>::elem*)f.data, f.norm, V};
+}
+
+// view of RHS `rhs` of a batched field (stride in complex elements; norms per V)
+template <int P>
+__device__ __forceinline__ SpinorView<P> rhs_view(SpinorView<P> v, int rhs, int64_t stride) {
+  if (rhs) {
+    if (v.data) v.data += rhs * stride;
+    if (v.norm) v.norm += (int64_t)rhs * v.V;
+  }
+  return v;
+}
+
 // ---------------------------------------------------------------------------
-// Epilogues.  apply() runs per site after the hop is computed; finalize()
-// runs on thread 0 of the last block with the grid totals of red[].
+// Epilogues.  at(rhs) returns the epilogue of one RHS; apply() runs per site
+// after the hop is computed and adds into red[] (FP64 block partials).
 // ---------------------------------------------------------------------------
 template <int P, int NCOMP>
 struct EpiStore {
   using R = typename Store<P>::R;
   static constexpr int NRED = 0;
   SpinorView<P> out;
-  EpiStore(const EpiArgs& a, int64_t V) : out{(typename Store<P>::elem*)a.out.data, a.out.norm, V} {}
+  int64_t stride;
+  EpiStore(const EpiArgs& a, int64_t V) : out(field_view<P>(a.out, V)), stride(a.rhs_stride) {}
+  __device__ EpiStore at(int rhs) const {
+    EpiStore e = *this;
+    e.out = rhs_view(out, rhs, stride);
+    return e;
+  }
   __device__ bool active() const { return true; }
   __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
                         double*) const {
@@ -37,7 +60,6 @@ struct EpiStore {
     wilson_finish<NCOMP, R>(lat, c, hop, res);
     out.template store<NCOMP>(s, res);
   }
-  __device__ void finalize(const double*) const {}
 };
 
 template <int P, int NCOMP>
@@ -46,10 +68,15 @@ struct EpiAxpy {
   static constexpr int NRED = 0;
   SpinorView<P> out, w;
   double sigma;
+  int64_t stride;
   EpiAxpy(const EpiArgs& a, int64_t V)
-      : out{(typename Store<P>::elem*)a.out.data, a.out.norm, V},
-        w{(typename Store<P>::elem*)a.aux0.data, a.aux0.norm, V},
-        sigma(a.sigma) {}
+      : out(field_view<P>(a.out, V)), w(field_view<P>(a.aux0, V)), sigma(a.sigma), stride(a.rhs_stride) {}
+  __device__ EpiAxpy at(int rhs) const {
+    EpiAxpy e = *this;
+    e.out = rhs_view(out, rhs, stride);
+    e.w = rhs_view(w, rhs, stride);
+    return e;
+  }
   __device__ bool active() const { return true; }
   __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
                         double*) const {
@@ -62,17 +89,23 @@ struct EpiAxpy {
     for (int k = 0; k < NCOMP; ++k) res[k] = mk<R>(res[k].re + sg * wv[k].re, res[k].im + sg * wv[k].im);
     out.template store<NCOMP>(s, res);
   }
-  __device__ void finalize(const double*) const {}
 };
 
-// CG-M step 1: t = Q p, reduce |t|^2 and |p|^2 (<p,(Q^2+s0)p> = |Qp|^2 + s0|p|^2).
+// CG step 1: t = Q p, reduce |t|^2 and |p|^2 (<p,(Q^2+s0)p> = |Qp|^2 + s0|p|^2).
 template <int P, int NCOMP>
 struct EpiCgmK1 {
   using R = typename Store<P>::R;
   static constexpr int NRED = 2;
   SpinorView<P> out;
   SolverState* st;
-  EpiCgmK1(const EpiArgs& a, int64_t V) : out{(typename Store<P>::elem*)a.out.data, a.out.norm, V}, st(a.st) {}
+  int64_t stride;
+  EpiCgmK1(const EpiArgs& a, int64_t V) : out(field_view<P>(a.out, V)), st(a.st), stride(a.rhs_stride) {}
+  __device__ EpiCgmK1 at(int rhs) const {
+    EpiCgmK1 e = *this;
+    e.out = rhs_view(out, rhs, stride);
+    e.st = st + rhs;
+    return e;
+  }
   // plain CG-M enters at PH_K1, the fast variant at PH_KP (after p = r + beta p)
   __device__ bool active() const { return st->phase == PH_K1 || st->phase == PH_KP; }
   __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
@@ -84,20 +117,25 @@ struct EpiCgmK1 {
     red[0] += norm2<NCOMP, R>(res);
     red[1] += norm2<NCOMP, R>(c);
   }
-  __device__ void finalize(const double* sums) const { cgm_fin_a(st, sums[0], sums[1]); }
 };
 
-// CG-M step 2: Ap = Q t + s0 p; r <- r - alpha Ap; reduce |r|^2.
+// Plain CG-M step 2: Ap = Q t + s0 p; r <- r - alpha Ap; reduce |r|^2.
 template <int P, int NCOMP>
 struct EpiCgmK2 {
   using R = typename Store<P>::R;
   static constexpr int NRED = 1;
   SpinorView<P> p, r;
   SolverState* st;
+  int64_t stride;
   EpiCgmK2(const EpiArgs& a, int64_t V)
-      : p{(typename Store<P>::elem*)a.aux0.data, a.aux0.norm, V},
-        r{(typename Store<P>::elem*)a.aux1.data, a.aux1.norm, V},
-        st(a.st) {}
+      : p(field_view<P>(a.aux0, V)), r(field_view<P>(a.aux1, V)), st(a.st), stride(a.rhs_stride) {}
+  __device__ EpiCgmK2 at(int rhs) const {
+    EpiCgmK2 e = *this;
+    e.p = rhs_view(p, rhs, stride);
+    e.r = rhs_view(r, rhs, stride);
+    e.st = st + rhs;
+    return e;
+  }
   __device__ bool active() const { return st->phase == PH_K2; }
   __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
                         double* red) const {
@@ -116,7 +154,6 @@ struct EpiCgmK2 {
     r.template store<NCOMP>(s, rv);
     red[0] += norm2<NCOMP, R>(rv);
   }
-  __device__ void finalize(const double* sums) const { cgm_fin_b(st, sums[0]); }
 };
 
 // Fast CG-M step 2: residuals live in a ring of nslot fields; iteration k reads
@@ -130,24 +167,32 @@ struct EpiCgmK2F {
   static constexpr int NRED = 1;
   SpinorView<P> p;
   E* ring;
-  float* ring_norm;
   int64_t slot_elems;
   int nslot;
   SolverState* st;
+  int64_t stride, ring_stride;
   EpiCgmK2F(const EpiArgs& a, int64_t V)
-      : p{(E*)a.aux0.data, a.aux0.norm, V},
+      : p(field_view<P>(a.aux0, V)),
         ring((E*)a.ring.data),
-        ring_norm(a.ring.norm),
         slot_elems(a.slot_elems),
         nslot(a.nslot),
-        st(a.st) {}
+        st(a.st),
+        stride(a.rhs_stride),
+        ring_stride(a.ring_rhs_stride) {}
+  __device__ EpiCgmK2F at(int rhs) const {
+    EpiCgmK2F e = *this;
+    e.p = rhs_view(p, rhs, stride);
+    e.ring = ring + rhs * ring_stride;
+    e.st = st + rhs;
+    return e;
+  }
   __device__ bool active() const { return st->phase == PH_K2; }
   __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
                         double* red) const {
     const int k = st->iter;
     const int si = k % nslot, so = (k + 1) % nslot;
-    const SpinorView<P> rin{ring + si * slot_elems, ring_norm ? ring_norm + si * lat.V : nullptr, lat.V};
-    const SpinorView<P> rout{ring + so * slot_elems, ring_norm ? ring_norm + so * lat.V : nullptr, lat.V};
+    const SpinorView<P> rin{ring + si * slot_elems, nullptr, lat.V};
+    const SpinorView<P> rout{ring + so * slot_elems, nullptr, lat.V};
     cpx<R> c[NCOMP], res[NCOMP], pv[NCOMP], rv[NCOMP];
     in.template load<NCOMP>(s, c);
     p.template load<NCOMP>(s, pv);
@@ -163,7 +208,6 @@ struct EpiCgmK2F {
     rout.template store<NCOMP>(s, rv);
     red[0] += norm2<NCOMP, R>(rv);
   }
-  __device__ void finalize(const double* sums) const { cgm_fin_b_fast(st, sums[0]); }
 };
 
 // MR step 1: t = Op r (phase gated).
@@ -173,7 +217,8 @@ struct EpiMrK1 {
   static constexpr int NRED = 0;
   SpinorView<P> out;
   SolverState* st;
-  EpiMrK1(const EpiArgs& a, int64_t V) : out{(typename Store<P>::elem*)a.out.data, a.out.norm, V}, st(a.st) {}
+  EpiMrK1(const EpiArgs& a, int64_t V) : out(field_view<P>(a.out, V)), st(a.st) {}
+  __device__ EpiMrK1 at(int) const { return *this; }
   __device__ bool active() const { return st->phase == PH_K1; }
   __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
                         double*) const {
@@ -182,7 +227,6 @@ struct EpiMrK1 {
     wilson_finish<NCOMP, R>(lat, c, hop, res);
     out.template store<NCOMP>(s, res);
   }
-  __device__ void finalize(const double*) const { st->phase = PH_K2; }
 };
 
 // MR step 2: p = Op t + s_N r; reduce <p,r> (complex) and <p,p>.
@@ -192,10 +236,8 @@ struct EpiMrK2 {
   static constexpr int NRED = 3;
   SpinorView<P> out, r;
   SolverState* st;
-  EpiMrK2(const EpiArgs& a, int64_t V)
-      : out{(typename Store<P>::elem*)a.out.data, a.out.norm, V},
-        r{(typename Store<P>::elem*)a.aux0.data, a.aux0.norm, V},
-        st(a.st) {}
+  EpiMrK2(const EpiArgs& a, int64_t V) : out(field_view<P>(a.out, V)), r(field_view<P>(a.aux0, V)), st(a.st) {}
+  __device__ EpiMrK2 at(int) const { return *this; }
   __device__ bool active() const { return st->phase == PH_K1; }
   __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
                         double* red) const {
@@ -214,38 +256,43 @@ struct EpiMrK2 {
     }
     out.template store<NCOMP>(s, res);
   }
-  __device__ void finalize(const double* sums) const { mr_fin(st, sums[0], sums[1], sums[2]); }
 };
 
 // ---------------------------------------------------------------------------
-// The kernel
+// The kernel.  gs = site blocks; the grid has gs * nrhs blocks.
 // ---------------------------------------------------------------------------
 template <int D, int NC, bool ADJ, bool PROJ, int P, class Epi>
-__global__ __launch_bounds__(kBlock) void k_wilson(Lat lat, SiteMap map, SpinorView<P> in,
+__global__ __launch_bounds__(kBlock) void k_wilson(Lat lat, SiteMap map, SpinorView<P> in, int64_t in_stride,
                                                    LinkView<typename Store<P>::R> lk,
-                                                   LinkView<typename Store<P>::R> lkr, Epi epi, RedCtl rc) {
+                                                   LinkView<typename Store<P>::R> lkr, Epi epi, RedCtl rc, int nrhs,
+                                                   int gs) {
   using R = typename Store<P>::R;
   constexpr int NCOMP = Gam<D>::NS * (ADJ ? NC * NC : NC);
-  if (!epi.active()) return;
+  int sblk = blockIdx.x, rhs = 0;
+  if (nrhs > 1) batch_block(nrhs, sblk, rhs);
+  const Epi e = epi.at(rhs);
+  if (!e.active()) return;
+  const SpinorView<P> inr = rhs_view(in, rhs, in_stride);
   constexpr int NR = Epi::NRED > 0 ? Epi::NRED : 1;
   double red[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) red[r] = 0.0;
   int64_t s;
   int x[D];
-  if (site_of_thread<D>(lat, map, s, x)) {
+  if (site_of_thread<D>(lat, map, sblk, gs, s, x)) {
     cpx<R> hop[NCOMP];
-    wilson_hop_x<D, NC, ADJ, PROJ, R>(lat, s, x, in, lk, lkr, hop);
-    epi.apply(lat, s, in, hop, red);
+    wilson_hop_x<D, NC, ADJ, PROJ, R>(lat, s, x, inr, lk, lkr, hop);
+    e.apply(lat, s, inr, hop, red);
   }
-  if constexpr (Epi::NRED > 0) block_partials<NR>(red, rc.partials);
+  if constexpr (Epi::NRED > 0) block_partials<NR>(red, rc.partials + (int64_t)rhs * NR * gs, sblk, gs);
 }
 
 // ---------------------------------------------------------------------------
 // Dispatch: runtime (kind, ndim, nc, rep, prec, mode) -> template instance.
 // ---------------------------------------------------------------------------
 template <int D, int NC, bool ADJ, int P, template <int, int> class EpiT>
-static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, RedCtl rc, hipStream_t stream) {
+static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, RedCtl rc, hipStream_t stream,
+                      int nrhs) {
   using R = typename Store<P>::R;
   constexpr int NCOMP = Gam<D>::NS * (ADJ ? NC * NC : NC);
   using Epi = EpiT<P, NCOMP>;
@@ -260,14 +307,14 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
   LinkView<R> lkr{(const cpx<R>*)lr, V};
   SpinorView<P> sv{(typename Store<P>::elem*)in.data, in.norm, V};
   Epi epi(a, V);
-  const dim3 grid(d.wblocks), block(d.wthreads);
+  const dim3 grid((unsigned)d.wblocks * (unsigned)nrhs), block(d.wthreads);
   if (proj) {
-    hipLaunchKernelGGL((k_wilson<D, NC, ADJ, true, P, Epi>), grid, block, 0, stream, d.lat, d.map, sv, lk, lkr, epi,
-                       rc);
+    hipLaunchKernelGGL((k_wilson<D, NC, ADJ, true, P, Epi>), grid, block, 0, stream, d.lat, d.map, sv, a.rhs_stride,
+                       lk, lkr, epi, rc, nrhs, d.wblocks);
   } else {
     if constexpr (P == P64 && std::is_same<Epi, EpiStore<P, NCOMP>>::value) {
-      hipLaunchKernelGGL((k_wilson<D, NC, ADJ, false, P, Epi>), grid, block, 0, stream, d.lat, d.map, sv, lk, lkr,
-                         epi, rc);
+      hipLaunchKernelGGL((k_wilson<D, NC, ADJ, false, P, Epi>), grid, block, 0, stream, d.lat, d.map, sv,
+                         a.rhs_stride, lk, lkr, epi, rc, nrhs, d.wblocks);
     } else {
       set_error("simple (full-projector) mode is only built for the plain complex128 apply");
       return LYM_ERR_UNSUPPORTED;
@@ -278,29 +325,30 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
 }
 
 template <int P, template <int, int> class EpiT>
-static int launch_geom(const OpDesc& d, bool proj, Field in, const EpiArgs& a, RedCtl rc, hipStream_t st) {
+static int launch_geom(const OpDesc& d, bool proj, Field in, const EpiArgs& a, RedCtl rc, hipStream_t st,
+                       int nrhs) {
   const int key = d.ndim * 100 + d.nc * 10 + d.adj;
   if constexpr (P == P16) {
     // half-precision storage is built for the fundamental representation only
     switch (key) {
-      case 420: return launch_one<4, 2, false, P, EpiT>(d, proj, in, a, rc, st);
-      case 430: return launch_one<4, 3, false, P, EpiT>(d, proj, in, a, rc, st);
-      case 220: return launch_one<2, 2, false, P, EpiT>(d, proj, in, a, rc, st);
-      case 230: return launch_one<2, 3, false, P, EpiT>(d, proj, in, a, rc, st);
+      case 420: return launch_one<4, 2, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      case 430: return launch_one<4, 3, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      case 220: return launch_one<2, 2, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      case 230: return launch_one<2, 3, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
       default:
         set_error("half-precision storage: fundamental representation only");
         return LYM_ERR_UNSUPPORTED;
     }
   }
   switch (key) {
-    case 420: return launch_one<4, 2, false, P, EpiT>(d, proj, in, a, rc, st);
-    case 430: return launch_one<4, 3, false, P, EpiT>(d, proj, in, a, rc, st);
-    case 220: return launch_one<2, 2, false, P, EpiT>(d, proj, in, a, rc, st);
-    case 230: return launch_one<2, 3, false, P, EpiT>(d, proj, in, a, rc, st);
-    case 421: return launch_one<4, 2, true, P, EpiT>(d, proj, in, a, rc, st);
-    case 431: return launch_one<4, 3, true, P, EpiT>(d, proj, in, a, rc, st);
-    case 221: return launch_one<2, 2, true, P, EpiT>(d, proj, in, a, rc, st);
-    case 231: return launch_one<2, 3, true, P, EpiT>(d, proj, in, a, rc, st);
+    case 420: return launch_one<4, 2, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+    case 430: return launch_one<4, 3, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+    case 220: return launch_one<2, 2, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+    case 230: return launch_one<2, 3, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+    case 421: return launch_one<4, 2, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+    case 431: return launch_one<4, 3, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+    case 221: return launch_one<2, 2, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+    case 231: return launch_one<2, 3, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
     default:
       set_error("unsupported (ndim, nc, representation)");
       return LYM_ERR_UNSUPPORTED;
@@ -308,11 +356,12 @@ static int launch_geom(const OpDesc& d, bool proj, Field in, const EpiArgs& a, R
 }
 
 template <template <int, int> class EpiT>
-static int launch_prec(int prec, const OpDesc& d, bool proj, Field in, const EpiArgs& a, RedCtl rc, hipStream_t st) {
+static int launch_prec(int prec, const OpDesc& d, bool proj, Field in, const EpiArgs& a, RedCtl rc, hipStream_t st,
+                       int nrhs) {
   switch (prec) {
-    case P64: return launch_geom<P64, EpiT>(d, proj, in, a, rc, st);
-    case P32: return launch_geom<P32, EpiT>(d, proj, in, a, rc, st);
-    case P16: return launch_geom<P16, EpiT>(d, proj, in, a, rc, st);
+    case P64: return launch_geom<P64, EpiT>(d, proj, in, a, rc, st, nrhs);
+    case P32: return launch_geom<P32, EpiT>(d, proj, in, a, rc, st, nrhs);
+    case P16: return launch_geom<P16, EpiT>(d, proj, in, a, rc, st, nrhs);
     default:
       set_error("unknown precision");
       return LYM_ERR_ARG;
@@ -320,21 +369,25 @@ static int launch_prec(int prec, const OpDesc& d, bool proj, Field in, const Epi
 }
 
 int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, const EpiArgs& a, RedCtl rc,
-                  hipStream_t stream) {
+                  hipStream_t stream, int nrhs) {
+  if (nrhs < 1) {
+    set_error("batch size must be >= 1");
+    return LYM_ERR_ARG;
+  }
   switch (kind) {
-    case EPI_STORE: return launch_prec<EpiStore>(prec, d, proj, in, a, rc, stream);
-    case EPI_AXPY: return launch_prec<EpiAxpy>(prec, d, proj, in, a, rc, stream);
-    case EPI_CGM_K1: return launch_prec<EpiCgmK1>(prec, d, proj, in, a, rc, stream);
-    case EPI_CGM_K2: return launch_prec<EpiCgmK2>(prec, d, proj, in, a, rc, stream);
-    case EPI_CGM_K2F: return launch_prec<EpiCgmK2F>(prec, d, proj, in, a, rc, stream);
+    case EPI_STORE: return launch_prec<EpiStore>(prec, d, proj, in, a, rc, stream, nrhs);
+    case EPI_AXPY: return launch_prec<EpiAxpy>(prec, d, proj, in, a, rc, stream, nrhs);
+    case EPI_CGM_K1: return launch_prec<EpiCgmK1>(prec, d, proj, in, a, rc, stream, nrhs);
+    case EPI_CGM_K2: return launch_prec<EpiCgmK2>(prec, d, proj, in, a, rc, stream, nrhs);
+    case EPI_CGM_K2F: return launch_prec<EpiCgmK2F>(prec, d, proj, in, a, rc, stream, nrhs);
     case EPI_MR_K1:
     case EPI_MR_K2:
-      if (prec != P64) {
-        set_error("the multishift MR solver runs in complex128 only");
+      if (prec != P64 || nrhs != 1) {
+        set_error("the multishift MR solver runs in complex128, one RHS per call");
         return LYM_ERR_UNSUPPORTED;
       }
-      if (kind == EPI_MR_K1) return launch_geom<P64, EpiMrK1>(d, proj, in, a, rc, stream);
-      return launch_geom<P64, EpiMrK2>(d, proj, in, a, rc, stream);
+      if (kind == EPI_MR_K1) return launch_geom<P64, EpiMrK1>(d, proj, in, a, rc, stream, 1);
+      return launch_geom<P64, EpiMrK2>(d, proj, in, a, rc, stream, 1);
   }
   set_error("unknown epilogue");
   return LYM_ERR_ARG;

@@ -240,13 +240,15 @@ __host__ __device__ __forceinline__ void wilson_hop(const Lat& lat, int64_t s, c
 }
 
 // Site and coordinates of this thread under the block schedule `m`; false if
-// the thread has no site (tail of a linear schedule).
+// the thread has no site (tail of a linear schedule).  `sblk` is the site
+// block's dispatch index among `gs` site blocks (sblk % kNumXcd = its XCD).
 template <int D>
-__device__ __forceinline__ bool site_of_thread(const Lat& lat, const SiteMap& m, int64_t& s, int (&x)[D]) {
+__device__ __forceinline__ bool site_of_thread(const Lat& lat, const SiteMap& m, int sblk, int gs, int64_t& s,
+                                               int (&x)[D]) {
   if constexpr (D == 4) {
     if (m.tiled) {
-      const int xcd = blockIdx.x % kNumXcd;
-      const int j = blockIdx.x / kNumXcd;
+      const int xcd = sblk % kNumXcd;
+      const int j = sblk / kNumXcd;
       const int x0l = j % m.slab;
       const int r = j / m.slab;
       const int g = r % m.ng;
@@ -258,10 +260,22 @@ __device__ __forceinline__ bool site_of_thread(const Lat& lat, const SiteMap& m,
       return true;
     }
   }
-  s = xcd_block() * blockDim.x + threadIdx.x;
+  const int64_t per = gs / kNumXcd;
+  const int64_t lb = (int64_t)(sblk % kNumXcd) * per + sblk / kNumXcd;
+  s = lb * blockDim.x + threadIdx.x;
   if (s >= lat.V) return false;
   coords<D>(lat, s, x);
   return true;
+}
+
+// (site block, right-hand side) of this block in a batched launch: all RHS of
+// one site block run back to back on the same XCD (they share its links).
+__device__ __forceinline__ void batch_block(int nrhs, int& sblk, int& rhs) {
+  const int b = blockIdx.x;
+  const int xcd = b % kNumXcd;
+  const int j = b / kNumXcd;
+  rhs = j % nrhs;
+  sblk = (j / nrhs) * kNumXcd + xcd;
 }
 
 // res = g5^herm (center + kappa * hop)

@@ -86,6 +86,38 @@ class multishift_solver:
                 self._cgm_generic(operator, src, shifts, i0, solutions)
         return solutions
 
+    def solve_batch(self, operator, sources, shifts):
+        """Multishift CG for several sources sharing one shift list, in the same
+        device launches (``algorithm="cg"``; new, the reference solves one
+        source per call).  Returns one list of solutions per source, in
+        ``shifts`` order; ``last_stats`` becomes a list (one dict per source)."""
+        w = dirac.wilson_of_square(operator)
+        if w is None or not w.hermitian:
+            return [self.solve(operator, s, shifts) for s in sources]
+        shifts = [float(np.real(s)) for s in shifts]
+        if any(s < 0 for s in shifts):
+            raise RuntimeError("batched CG-M needs non-negative shifts")
+        srcs = [to_device(s) for s in sources]
+        nrhs, n = len(srcs), len(shifts)
+        lib = _lib.lib()
+        prec = _PREC[self.precision_mode]
+        if prec != _lib.PREC_F64:
+            w.ensure_f32()
+        desc = w.desc
+        nbytes = lib.lym_cgm_batched_workspace_bytes(ctypes.byref(desc), prec, n, nrhs)
+        ws = _workspace(nbytes, srcs[0].device)
+        sols = [[torch.empty_like(srcs[0]) for _ in shifts] for _ in srcs]
+        sh = (ctypes.c_double * n)(*shifts)
+        bs = (ctypes.c_void_p * nrhs)(*[s.data_ptr() for s in srcs])
+        xs = (ctypes.c_void_p * (nrhs * n))(*[x.data_ptr() for row in sols for x in row])
+        st = (_lib.LymSolveStats * nrhs)()
+        rc = _lib.check(lib.lym_cgm_batched(ctypes.byref(desc), prec, nrhs, n, sh, bs, xs, float(self.tolerance),
+                                            int(self.max_steps), _lib.ptr(ws), ws.numel(), st,
+                                            _lib.stream_handle(srcs[0].device)), "cgm_batched")
+        self.last_stats = [st[i].as_dict(n) for i in range(nrhs)]
+        self.last_status = _status_name(rc)
+        return sols
+
     # ---- device paths ------------------------------------------------------
     def _mr_device(self, w, src, shifts, i0, solutions):
         n = len(shifts)
