@@ -68,6 +68,61 @@ def test_tiled_schedule_parity(geometry, nc, rep, bc):
     assert rel(op.apply(psi), want) < TOL64
 
 
+def _tile_shapes(geometry, es, ncomp):
+    """Every LDS tile (a0, a1, k) the library admits on `geometry`: a0 divides
+    L0/8, a1 | L1, k | L2, a0*a1*k*L3 a multiple of 64 and <= 640 sites,
+    tile spinor <= 64 KiB."""
+    L0, L1, L2, T = geometry
+    slab = L0 // 8
+    out = []
+    for a0 in range(1, slab + 1):
+        for a1 in range(1, L1 + 1):
+            for k in range(1, L2 + 1):
+                ns = a0 * a1 * k * T
+                if slab % a0 or L1 % a1 or L2 % k or ns % 64 or ns > 640 or ns * ncomp * es > 65536:
+                    continue
+                out.append((a0, a1, k))
+    return out
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("geometry", [(16, 8, 8, 16), (8, 4, 6, 32)])
+def test_lds_tile_shapes(prec, geometry, monkeypatch):
+    """Every admissible LDS tile shape (forced through LYM_TILE) gives the
+    oracle's D, and the same result as the untiled kernel (LYM_NO_TILE=1) to
+    rounding: partial tiles, full-extent wrap, tile edges in every direction."""
+    import ctypes
+    from tfleonardym_amd import _lib
+    print(f"case lds tiles prec={prec} geometry={geometry}", flush=True)
+    U = fields.gauge_field(geometry, 3, seed=41, kind="haar")
+    psi = fields.gaussian_spinor(geometry, 4, 3, seed=42)
+    op = make_op(U, geometry, 0.127, True, "fundamental", -1.0)
+    desc = op.ensure_f32()
+    want = ow.WilsonOracle(U, geometry, 0.127, True, "fundamental", bc_factor=-1.0).apply(psi)
+    pc = _lib.PREC_F64 if prec == "f64" else _lib.PREC_F32
+    dt = torch.complex128 if prec == "f64" else torch.complex64
+    x = torch.from_numpy(psi).to("cuda").to(dt)
+
+    def run():
+        out = torch.zeros_like(x)
+        _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), pc, 0, _lib.ptr(x), _lib.ptr(out),
+                                               _lib.stream_handle()))
+        return out
+
+    monkeypatch.setenv("LYM_NO_TILE", "1")
+    plain = run()
+    monkeypatch.setenv("LYM_NO_TILE", "0")
+    shapes = _tile_shapes(geometry, 16 if prec == "f64" else 8, 12)
+    assert shapes
+    tol = TOL64 if prec == "f64" else TOL32
+    for a0, a1, k in shapes:
+        monkeypatch.setenv("LYM_TILE", f"{a0},{a1},{k}")
+        got = run()
+        assert rel(got.to(torch.complex128), want) < tol, (a0, a1, k)
+        d = (got - plain).abs().max().item() / plain.abs().max().item()
+        assert d < (1e-15 if prec == "f64" else 1e-6), (a0, a1, k, d)
+
+
 def test_custom_provider_is_materialised():
     """A provider that is not the standard one is applied once to the links."""
     from tfleonardym_amd import boundary_conditions, dirac, lattice

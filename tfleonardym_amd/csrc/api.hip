@@ -1,6 +1,9 @@
 // extern "C" entry points of liblym.so (declared in include/lym.h).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -96,6 +99,53 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
         }
       }
     }
+  }
+  // LYM_NO_TILE=1 in the environment forces the plain kernel (A/B checks);
+  // LYM_TILE=a0,a1,k forces a tile shape when it is admissible
+  const char* nt = std::getenv("LYM_NO_TILE");
+  o.use_tile = (nt && nt[0] == '1') ? 0 : 1;
+  int want[3] = {0, 0, 0};
+  if (const char* wt = std::getenv("LYM_TILE")) {
+    if (std::sscanf(wt, "%d,%d,%d", &want[0], &want[1], &want[2]) != 3) want[0] = want[1] = want[2] = 0;
+  }
+  // LDS tile per precision (fundamental 4D, L0 a multiple of the XCD count):
+  // minimise global spinor reads per site, 1 + sum_{mu<3} 2/a_mu (0 for a
+  // full extent), within 64 KiB of LDS and 640 threads (a multiple of 64).
+  // Blocks of 256..512 sites are preferred when the lattice admits them
+  // (smaller tiles re-read more, larger ones lose occupancy); ties go to the
+  // deeper x2 extent (neighbours nearer in memory), then the smaller tile.
+  for (int pi = 0; pi < 2; ++pi) {
+    const int64_t es = pi == 0 ? 16 : 8;
+    TileMap best{};
+    if (w->ndim == 4 && !o.adj && w->dims[0] % kNumXcd == 0) {
+      const int L0 = w->dims[0], L1 = w->dims[1], L2 = w->dims[2], T = w->dims[3];
+      const int slab = L0 / kNumXcd;
+      auto part = [](int a, int L) { return a == L ? 0.0 : 2.0 / a; };
+      for (int pass = 0; pass < 2 && !best.on; ++pass) {
+        double best_cost = 1e30;
+        for (int a0 = 1; a0 <= slab; ++a0) {
+          if (slab % a0) continue;
+          for (int a1 = 1; a1 <= L1; ++a1) {
+            if (L1 % a1) continue;
+            for (int k = 1; k <= L2; ++k) {
+              if (L2 % k) continue;
+              const int64_t ns = (int64_t)a0 * a1 * k * T;
+              if (ns % 64 || ns > kTileMaxThreads) continue;
+              if (ns * o.ncomp * es > kTileMaxLds) continue;
+              const bool forced = want[0] && a0 == want[0] && a1 == want[1] && k == want[2];
+              if (pass == 0 && !forced && (ns < 256 || ns > 512)) continue;
+              const double cost = forced ? -1.0 : 1.0 + part(a0, L0) + part(a1, L1) + part(k, L2);
+              const bool tie = std::fabs(cost - best_cost) < 1e-9;
+              if (cost < best_cost - 1e-9 || (tie && (k > best.k || (k == best.k && ns < best.nsite)))) {
+                best_cost = cost;
+                best = TileMap{1, a0, a1, k, T, slab, slab / a0, L1 / a1, L2 / k, (int)ns, (int)(V / ns)};
+              }
+            }
+          }
+        }
+      }
+    }
+    o.tile[pi] = best;
   }
   *d = o;
   return LYM_OK;

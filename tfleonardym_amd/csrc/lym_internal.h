@@ -23,9 +23,11 @@ struct OpDesc {
   int ndim = 0, nc = 0, adj = 0, herm = 0;
   int ns = 0, ncc = 0, ncomp = 0;  // spins, colour components per spin, total
   Lat lat{};
-  SiteMap map{};     // block schedule of the Wilson kernels
-  int wblocks = 0;   // Wilson grid size (blocks)
-  int wthreads = 0;  // Wilson block size (threads, multiple of 64)
+  SiteMap map{};     // block schedule of the plain Wilson kernel
+  int wblocks = 0;   // plain Wilson grid size (blocks)
+  int wthreads = 0;  // plain Wilson block size (threads, multiple of 64)
+  TileMap tile[2]{};  // LDS-tiled kernel per precision: [0] complex128, [1] complex64
+  int use_tile = 1;   // 0 forces the plain kernel
   const void* links64 = nullptr;
   const void* linksr64 = nullptr;
   const void* links32 = nullptr;
@@ -73,6 +75,13 @@ enum EpiKind : int {
 };
 int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, const EpiArgs& a, RedCtl rc,
                   hipStream_t stream, int nrhs = 1);
+// number of site blocks (= partials per RHS) of a Wilson launch at `prec`
+inline bool wilson_tiled(const OpDesc& d, int prec) {
+  return d.use_tile && (prec == P64 || prec == P32) && d.tile[prec == P64 ? 0 : 1].on;
+}
+inline int wilson_blocks(const OpDesc& d, int prec) {
+  return wilson_tiled(d, prec) ? d.tile[prec == P64 ? 0 : 1].blocks : d.wblocks;
+}
 
 // ---- finalizer kernel (solvers.hip): one block reduces the partials of the
 // preceding kernel in a fixed order, then advances the solver state.
@@ -108,5 +117,7 @@ inline int64_t max_red_blocks(int64_t V) {
   return g > 2048 ? g : 2048;
 }
 constexpr int kMaxRed = 4;  // most values any kernel reduces
+constexpr int kTileMaxThreads = 640;        // LDS-tiled Wilson block size limit
+constexpr int64_t kTileMaxLds = 64 * 1024;  // LDS bytes per tiled block
 
 }  // namespace lym

@@ -56,12 +56,13 @@ struct RedCtl {
 };
 
 // Block sums of v[NR] -> partials[r * G + idx] (thread 0 writes); by default
-// G = gridDim.x, idx = blockIdx.x.  blockDim.x must be a multiple of 64, <= kBlock.
-template <int NR>
+// G = gridDim.x, idx = blockIdx.x.  blockDim.x must be a multiple of 64,
+// <= 64 * MAXW.
+template <int NR, int MAXW = kBlock / 64>
 __device__ __forceinline__ void block_partials(const double (&v)[NR], double* partials, int idx = -1, int G = -1) {
   if (idx < 0) idx = blockIdx.x;
   if (G < 0) G = gridDim.x;
-  __shared__ double sm[NR][kBlock / 64];
+  __shared__ double sm[NR][MAXW];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int nw = blockDim.x >> 6;

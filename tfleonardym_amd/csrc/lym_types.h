@@ -220,6 +220,12 @@ struct SpinorView {
       }
     }
   }
+  // neighbour load of a hop (mu, dir 0 = forward, 1 = backward): global here;
+  // the LDS tile view overrides it
+  template <int NCOMP>
+  __host__ __device__ __forceinline__ void load_hop(int, int, int64_t s, cpx<R> (&v)[NCOMP]) const {
+    load<NCOMP>(s, v);
+  }
   template <int NCOMP>
   __host__ __device__ __forceinline__ void store(int64_t s, const cpx<R> (&v)[NCOMP]) const {
     const uint32_t bytes = (uint32_t)(NCOMP * V * (int64_t)sizeof(E));
@@ -286,6 +292,20 @@ struct SiteMap {
   int k;     // lines per block
   int slab;  // x0 values per XCD
   int ng;    // L_2 / k
+};
+
+// LDS-tiled Wilson kernel (4D): a block stages the input spinor of a tile of
+// a0 x a1 x k x T sites (whole lines of the fastest direction) in LDS and
+// serves every hop whose neighbour is inside the tile from there.  XCD x owns
+// x0 in [x*slab, (x+1)*slab); within an XCD tiles run x0 fastest, then x2,
+// then x1.  on = 0: not applicable (the plain kernel is used).
+struct TileMap {
+  int on;
+  int a0, a1, k, T;
+  int slab;
+  int n0, n1, n2;  // tiles per XCD along x0, x1, x2
+  int nsite;       // a0 * a1 * k * T (block size)
+  int blocks;      // tiles in the lattice
 };
 
 template <int D>

@@ -606,7 +606,7 @@ static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Fi
   LYM_HIP(hipMemcpyAsync(P.xs_dev, hx, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
   LYM_HIP(hipMemcpyAsync(P.ps_dev, hp, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
   const int64_t n = field_elems(d);
-  const int gw = d.wblocks;  // Wilson grid: number of partials its reductions write
+  const int gw = wilson_blocks(d, prec);  // Wilson grid: number of partials its reductions write
   const int gr = red_grid_flat(n);
   const int gf = flat_grid(n);
   if (prec == P64)
@@ -684,7 +684,6 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   LYM_HIP(hipMemcpyAsync(P.xs_dev, hx.data(), sizeof(Field) * hx.size(), hipMemcpyHostToDevice, st));
   LYM_HIP(hipMemcpyAsync(P.ps_dev, hp.data(), sizeof(Field) * hp.size(), hipMemcpyHostToDevice, st));
   LYM_HIP(hipMemcpyAsync(P.bs_dev, hb.data(), sizeof(Field) * nrhs, hipMemcpyHostToDevice, st));
-  const int gw = d.wblocks;
   const int gr = red_grid_flat(n);
   const int gf = flat_grid(n);
   const int nslot = kFastBlock + 1;
@@ -700,6 +699,7 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
     const Field ring = prec == P64 ? P.ring64 : P.ring32;
     const Field pl = prec == P64 ? P.plive64 : P.plive32;
     const Field t = prec == P64 ? P.tb64 : P.tb32;
+    const int gw = wilson_blocks(d, prec);  // Wilson partials per RHS at this precision
     for (int j = 0; j < kFastBlock; ++j) {
       if (prec == P64)
         hipLaunchKernelGGL(k_fast_kp<P64>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const double2*)ring.data,
@@ -967,7 +967,7 @@ int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const
   LYM_HIP(hipMemcpyAsync(P.st, &h, sizeof(h), hipMemcpyHostToDevice, st));
   LYM_HIP(hipMemcpyAsync(P.xs_dev, hx, sizeof(Field) * nshift, hipMemcpyHostToDevice, st));
   const int64_t n = field_elems(d);
-  const int gw = d.wblocks;  // Wilson grid: number of partials its reductions write
+  const int gw = wilson_blocks(d, P64);  // Wilson grid: number of partials its reductions write
   const int gr = red_grid_flat(n);
   const int gf = flat_grid(n);
   hipLaunchKernelGGL(k_mr_init, dim3(gf), dim3(kBlock), 0, st, n, (const double2*)b, (double2*)P.r64.data, P.xs_dev,

@@ -38,7 +38,9 @@ __device__ __forceinline__ SpinorView<P> rhs_view(SpinorView<P> v, int rhs, int6
 
 // ---------------------------------------------------------------------------
 // Epilogues.  at(rhs) returns the epilogue of one RHS; apply() runs per site
-// after the hop is computed and adds into red[] (FP64 block partials).
+// after the hop is computed and adds into red[] (FP64 block partials).  `in`
+// supplies the center value of the site through load(s): the global field in
+// the plain kernel, the LDS tile in the tiled one.
 // ---------------------------------------------------------------------------
 template <int P, int NCOMP>
 struct EpiStore {
@@ -53,7 +55,8 @@ struct EpiStore {
     return e;
   }
   __device__ bool active() const { return true; }
-  __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
                         double*) const {
     cpx<R> c[NCOMP], res[NCOMP];
     in.template load<NCOMP>(s, c);
@@ -78,7 +81,8 @@ struct EpiAxpy {
     return e;
   }
   __device__ bool active() const { return true; }
-  __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
                         double*) const {
     cpx<R> c[NCOMP], res[NCOMP], wv[NCOMP];
     in.template load<NCOMP>(s, c);
@@ -108,7 +112,8 @@ struct EpiCgmK1 {
   }
   // plain CG-M enters at PH_K1, the fast variant at PH_KP (after p = r + beta p)
   __device__ bool active() const { return st->phase == PH_K1 || st->phase == PH_KP; }
-  __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
                         double* red) const {
     cpx<R> c[NCOMP], res[NCOMP];
     in.template load<NCOMP>(s, c);
@@ -137,7 +142,8 @@ struct EpiCgmK2 {
     return e;
   }
   __device__ bool active() const { return st->phase == PH_K2; }
-  __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
                         double* red) const {
     cpx<R> c[NCOMP], res[NCOMP], pv[NCOMP], rv[NCOMP];
     in.template load<NCOMP>(s, c);
@@ -187,7 +193,8 @@ struct EpiCgmK2F {
     return e;
   }
   __device__ bool active() const { return st->phase == PH_K2; }
-  __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
                         double* red) const {
     const int k = st->iter;
     const int si = k % nslot, so = (k + 1) % nslot;
@@ -220,7 +227,8 @@ struct EpiMrK1 {
   EpiMrK1(const EpiArgs& a, int64_t V) : out(field_view<P>(a.out, V)), st(a.st) {}
   __device__ EpiMrK1 at(int) const { return *this; }
   __device__ bool active() const { return st->phase == PH_K1; }
-  __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
                         double*) const {
     cpx<R> c[NCOMP], res[NCOMP];
     in.template load<NCOMP>(s, c);
@@ -239,7 +247,8 @@ struct EpiMrK2 {
   EpiMrK2(const EpiArgs& a, int64_t V) : out(field_view<P>(a.out, V)), r(field_view<P>(a.aux0, V)), st(a.st) {}
   __device__ EpiMrK2 at(int) const { return *this; }
   __device__ bool active() const { return st->phase == PH_K1; }
-  __device__ void apply(const Lat& lat, int64_t s, const SpinorView<P>& in, const cpx<R> (&hop)[NCOMP],
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
                         double* red) const {
     cpx<R> c[NCOMP], res[NCOMP], rv[NCOMP];
     in.template load<NCOMP>(s, c);
@@ -288,6 +297,141 @@ __global__ __launch_bounds__(kBlock) void k_wilson(Lat lat, SiteMap map, SpinorV
 }
 
 // ---------------------------------------------------------------------------
+// LDS-tiled kernel (4D, complex128/complex64): a block owns a tile of
+// a0 x a1 x k x T sites (T = L_3, whole lines), one site per thread, thread
+// id = ((t0 * a1 + t1) * k + t2) * T + t3.  The block first copies the input
+// spinor of its tile into LDS (one coalesced read per site), then every hop
+// whose neighbour lies inside the tile reads LDS; only hops that leave the
+// tile read global memory.  Global spinor reads per site drop from 1 + 2D to
+// 1 + sum_{mu<3} 2/a_mu (a direction the tile spans completely costs none).
+// ---------------------------------------------------------------------------
+template <int P>
+struct TileView {
+  using R = typename Store<P>::R;
+  using E = typename Store<P>::elem;
+  SpinorView<P> g;  // the field in global memory (hops leaving the tile)
+  const E* lds;     // [NCOMP][nsite]
+  int nsite, self;  // tile size, this thread's tile index
+  int t[4];         // tile coordinates of this thread
+  int a[4];         // tile extents
+  int str[4];       // tile index strides
+  bool full[4];     // the tile spans the whole lattice in mu (wrap inside)
+
+  // center value: this thread's own site
+  template <int NCOMP>
+  __device__ __forceinline__ void load(int64_t, cpx<R> (&v)[NCOMP]) const {
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) {
+      const E e = lds[c * nsite + self];
+      v[c] = mk<R>(e.x, e.y);
+    }
+  }
+  // neighbour in direction mu (dir 0: +mu, 1: -mu); nb is its global site
+  template <int NCOMP>
+  __device__ __forceinline__ void load_hop(int mu, int dir, int64_t nb, cpx<R> (&v)[NCOMP]) const {
+    int tc = t[mu] + (dir == 0 ? 1 : -1);
+    bool inside = true;
+    if (tc < 0) {
+      tc += a[mu];
+      inside = full[mu];
+    } else if (tc >= a[mu]) {
+      tc -= a[mu];
+      inside = full[mu];
+    }
+    if (inside) {
+      const int idx = self + (tc - t[mu]) * str[mu];
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        const E e = lds[c * nsite + idx];
+        v[c] = mk<R>(e.x, e.y);
+      }
+    } else {
+      g.template load<NCOMP>(nb, v);
+    }
+  }
+};
+
+template <int NC, bool ADJ, int P, class Epi>
+__global__ __launch_bounds__(kTileMaxThreads) void k_wilson_tile(Lat lat, TileMap tm, SpinorView<P> in,
+                                                                 int64_t in_stride,
+                                                                 LinkView<typename Store<P>::R> lk,
+                                                                 LinkView<typename Store<P>::R> lkr, Epi epi,
+                                                                 RedCtl rc, int nrhs) {
+  using R = typename Store<P>::R;
+  using E = typename Store<P>::elem;
+  constexpr int NCOMP = Gam<4>::NS * (ADJ ? NC * NC : NC);
+  extern __shared__ __align__(16) unsigned char tile_smem[];
+  int tblk = blockIdx.x, rhs = 0;
+  if (nrhs > 1) batch_block(nrhs, tblk, rhs);
+  const Epi e = epi.at(rhs);
+  if (!e.active()) return;  // uniform per block (state of one RHS)
+  const SpinorView<P> inr = rhs_view(in, rhs, in_stride);
+  // tile origin: XCD slab, then x0 fastest, x2, x1
+  const int xcd = tblk % kNumXcd;
+  const int j = tblk / kNumXcd;
+  const int i0 = j % tm.n0;
+  const int r = j / tm.n0;
+  const int i2 = r % tm.n2;
+  const int i1 = r / tm.n2;
+  const int tid = (int)threadIdx.x;
+  int q = tid / tm.T;
+  const int t3 = tid - q * tm.T;
+  const int t2 = q % tm.k;
+  q /= tm.k;
+  const int t1 = q % tm.a1;
+  const int t0 = q / tm.a1;
+  int x[4];
+  x[0] = xcd * tm.slab + i0 * tm.a0 + t0;
+  x[1] = i1 * tm.a1 + t1;
+  x[2] = i2 * tm.k + t2;
+  x[3] = t3;
+  const int64_t s = (((int64_t)x[0] * lat.L[1] + x[1]) * lat.L[2] + x[2]) * tm.T + x[3];
+  E* lds = reinterpret_cast<E*>(tile_smem);
+  {
+    cpx<R> c[NCOMP];
+    inr.template load<NCOMP>(s, c);
+#pragma unroll
+    for (int k = 0; k < NCOMP; ++k) {
+      E v;
+      v.x = c[k].re;
+      v.y = c[k].im;
+      lds[k * tm.nsite + tid] = v;
+    }
+  }
+  __syncthreads();
+  TileView<P> view;
+  view.g = inr;
+  view.lds = lds;
+  view.nsite = tm.nsite;
+  view.self = tid;
+  view.t[0] = t0;
+  view.t[1] = t1;
+  view.t[2] = t2;
+  view.t[3] = t3;
+  view.a[0] = tm.a0;
+  view.a[1] = tm.a1;
+  view.a[2] = tm.k;
+  view.a[3] = tm.T;
+  view.str[3] = 1;
+  view.str[2] = tm.T;
+  view.str[1] = tm.k * tm.T;
+  view.str[0] = tm.a1 * tm.k * tm.T;
+  view.full[0] = tm.a0 == lat.L[0];
+  view.full[1] = tm.a1 == lat.L[1];
+  view.full[2] = tm.k == lat.L[2];
+  view.full[3] = true;
+  constexpr int NR = Epi::NRED > 0 ? Epi::NRED : 1;
+  double red[NR];
+#pragma unroll
+  for (int i = 0; i < NR; ++i) red[i] = 0.0;
+  cpx<R> hop[NCOMP];
+  wilson_hop_x<4, NC, ADJ, true, R>(lat, s, x, view, lk, lkr, hop);
+  e.apply(lat, s, view, hop, red);
+  if constexpr (Epi::NRED > 0)
+    block_partials<NR, kTileMaxThreads / 64>(red, rc.partials + (int64_t)rhs * NR * tm.blocks, tblk, tm.blocks);
+}
+
+// ---------------------------------------------------------------------------
 // Dispatch: runtime (kind, ndim, nc, rep, prec, mode) -> template instance.
 // ---------------------------------------------------------------------------
 template <int D, int NC, bool ADJ, int P, template <int, int> class EpiT>
@@ -307,6 +451,17 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
   LinkView<R> lkr{(const cpx<R>*)lr, V};
   SpinorView<P> sv{(typename Store<P>::elem*)in.data, in.norm, V};
   Epi epi(a, V);
+  if constexpr (D == 4 && P != P16 && !ADJ) {
+    if (proj && wilson_tiled(d, P)) {
+      const TileMap& tm = d.tile[P == P64 ? 0 : 1];
+      const size_t lds = (size_t)NCOMP * tm.nsite * sizeof(typename Store<P>::elem);
+      const dim3 grid((unsigned)tm.blocks * (unsigned)nrhs), block(tm.nsite);
+      hipLaunchKernelGGL((k_wilson_tile<NC, ADJ, P, Epi>), grid, block, lds, stream, d.lat, tm, sv, a.rhs_stride, lk,
+                         lkr, epi, rc, nrhs);
+      LYM_HIP(hipGetLastError());
+      return LYM_OK;
+    }
+  }
   const dim3 grid((unsigned)d.wblocks * (unsigned)nrhs), block(d.wthreads);
   if (proj) {
     hipLaunchKernelGGL((k_wilson<D, NC, ADJ, true, P, Epi>), grid, block, 0, stream, d.lat, d.map, sv, a.rhs_stride,

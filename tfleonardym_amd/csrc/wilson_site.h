@@ -160,7 +160,7 @@ __host__ __device__ __forceinline__ void wilson_hop_x(const Lat& lat, int64_t s,
       hop_commit<NCOMP, R>(acc);
       hop_dep<R>(nb, ls, acc[0].re);
       cpx<R> v[NCOMP];
-      psi.template load<NCOMP>(nb, v);
+      psi.template load_hop<NCOMP>(mu, dir, nb, v);
       cpx<R> u[NC][NC];
       links.template load<NC>(mu, ls, u);
       cpx<R> ur[NC][NC];
