@@ -115,12 +115,16 @@ def test_lds_tile_shapes(prec, geometry, monkeypatch):
     shapes = _tile_shapes(geometry, 16 if prec == "f64" else 8, 12)
     assert shapes
     tol = TOL64 if prec == "f64" else TOL32
-    for a0, a1, k in shapes:
+    for i, (a0, a1, k) in enumerate(shapes):
         monkeypatch.setenv("LYM_TILE", f"{a0},{a1},{k}")
-        got = run()
-        assert rel(got.to(torch.complex128), want) < tol, (a0, a1, k)
-        d = (got - plain).abs().max().item() / plain.abs().max().item()
-        assert d < (1e-15 if prec == "f64" else 1e-6), (a0, a1, k, d)
+        # hops in flight per wave: every value on the first and last shapes
+        for hif in ((1, 2, 4) if i in (0, len(shapes) - 1) else (1,)):
+            monkeypatch.setenv("LYM_HIF", str(hif))
+            got = run()
+            assert rel(got.to(torch.complex128), want) < tol, (a0, a1, k, hif)
+            d = (got - plain).abs().max().item() / plain.abs().max().item()
+            assert d < (1e-15 if prec == "f64" else 1e-6), (a0, a1, k, hif, d)
+        monkeypatch.delenv("LYM_HIF")
 
 
 def test_custom_provider_is_materialised():

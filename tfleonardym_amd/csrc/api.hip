@@ -130,7 +130,7 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
             for (int k = 1; k <= L2; ++k) {
               if (L2 % k) continue;
               const int64_t ns = (int64_t)a0 * a1 * k * T;
-              if (ns % 64 || ns > kTileMaxThreads) continue;
+              if (ns % 64 || ns > (pi == 0 ? kTileMaxThreads64 : kTileMaxThreads)) continue;
               if (ns * o.ncomp * es > kTileMaxLds) continue;
               const bool forced = want[0] && a0 == want[0] && a1 == want[1] && k == want[2];
               if (pass == 0 && !forced && (ns < 256 || ns > 512)) continue;
@@ -144,6 +144,11 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
           }
         }
       }
+    }
+    best.hif = 1;
+    if (const char* hf = std::getenv("LYM_HIF")) {
+      const int h = std::atoi(hf);
+      if (h == 2 || h == 4) best.hif = h;
     }
     o.tile[pi] = best;
   }

@@ -118,6 +118,7 @@ inline int64_t max_red_blocks(int64_t V) {
 }
 constexpr int kMaxRed = 4;  // most values any kernel reduces
 constexpr int kTileMaxThreads = 640;        // LDS-tiled Wilson block size limit
+constexpr int kTileMaxThreads64 = 320;      // the same for complex128 (LDS bound)
 constexpr int64_t kTileMaxLds = 64 * 1024;  // LDS bytes per tiled block
 
 }  // namespace lym

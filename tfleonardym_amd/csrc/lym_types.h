@@ -306,6 +306,7 @@ struct TileMap {
   int n0, n1, n2;  // tiles per XCD along x0, x1, x2
   int nsite;       // a0 * a1 * k * T (block size)
   int blocks;      // tiles in the lattice
+  int hif;         // hops in flight per wave: 1, 2 or 4 (0 = 1)
 };
 
 template <int D>

@@ -307,23 +307,30 @@ __global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int64
 }
 
 // precision switch at a block boundary: low-precision copies of the current
-// residual (ring slot iter % nslot) and of the live direction
+// residual (FP64 ring slot iter % nslot64 -> low ring slot iter % nslot_lo)
+// and of the live direction; later blocks run mblk_lo iterations
 template <int PL>
 __global__ __launch_bounds__(kBlock) void k_fast_switch(SolverState* st, int64_t n, const double2* __restrict__ ring64,
-                                                        Elem<PL>* __restrict__ ringlo, int64_t ring_stride, int nslot,
-                                                        const double2* __restrict__ p64, Elem<PL>* __restrict__ plo) {
+                                                        int64_t ring_stride64, int nslot64,
+                                                        Elem<PL>* __restrict__ ringlo, int64_t ring_stride_lo,
+                                                        int nslot_lo, const double2* __restrict__ p64,
+                                                        Elem<PL>* __restrict__ plo, int mblk_lo) {
   const int rhs = blockIdx.y;
   st += rhs;
-  ring64 += rhs * ring_stride;
-  ringlo += rhs * ring_stride;
+  ring64 += rhs * ring_stride64;
+  ringlo += rhs * ring_stride_lo;
   p64 += rhs * n;
   plo += rhs * n;
-  const int64_t off = (int64_t)(st->iter % nslot) * n;
+  const int64_t off64 = (int64_t)(st->iter % nslot64) * n;
+  const int64_t offlo = (int64_t)(st->iter % nslot_lo) * n;
   LYM_GRID_STRIDE(i, n) {
-    ringlo[off + i] = from64<Elem<PL>>(ring64[off + i]);
+    ringlo[offlo + i] = from64<Elem<PL>>(ring64[off64 + i]);
     plo[i] = from64<Elem<PL>>(p64[i]);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->precision_phase = 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->precision_phase = 1;
+    st->mblk = mblk_lo;
+  }
 }
 
 // ---- MR (complex128) -------------------------------------------------------
@@ -417,7 +424,12 @@ static Field take_field(Carver& c, const OpDesc& d, int prec) {
   return f;
 }
 
-constexpr int kFastBlock = 8;  // iterations per deferred shift-update block
+// iterations per deferred shift-update block: short while the Krylov vectors
+// are FP64 (the precision switch waits for a block boundary), long in FP32
+// (each block reads and writes every shift's FP64 x_s, p_s once)
+constexpr int kFastBlock64 = 8;
+constexpr int kFastBlock32 = 16;
+static_assert(kFastBlock64 <= kMaxBlock && kFastBlock32 <= kMaxBlock, "block longer than the recorded history");
 
 struct Parts {
   SolverState* st;  // nrhs states
@@ -433,7 +445,7 @@ struct Parts {
   Field x_lo[kMaxShifts];
   Field p[kMaxShifts];  // plain CG-M search directions
   // fast CG-M (mixed), every field batched over nrhs RHS:
-  Field ring64, ring32;     // nrhs x (kFastBlock + 1) slots
+  Field ring64, ring32;     // nrhs x (kFastBlock64 + 1), nrhs x (kFastBlock32 + 1) slots
   Field plive64, plive32;   // live base direction
   Field tb64, tb32;         // t = Q p
   Field ps64;               // nrhs x nshift FP64 shift directions
@@ -464,8 +476,8 @@ static size_t carve(const OpDesc& d, int prec, int nshift, int nrhs, char* base,
     P.t_lo = take_field(c, d, P32);
     P.x_lo[0] = take_field(c, d, P32);
     P.p[0] = take_field(c, d, P32);
-    P.ring64.data = c.take(field_bytes(d, P64) * (kFastBlock + 1) * nrhs);
-    P.ring32.data = c.take(field_bytes(d, P32) * (kFastBlock + 1) * nrhs);
+    P.ring64.data = c.take(field_bytes(d, P64) * (kFastBlock64 + 1) * nrhs);
+    P.ring32.data = c.take(field_bytes(d, P32) * (kFastBlock32 + 1) * nrhs);
     P.plive64.data = c.take(field_bytes(d, P64) * nrhs);
     P.plive32.data = c.take(field_bytes(d, P32) * nrhs);
     P.tb64.data = c.take(field_bytes(d, P64) * nrhs);
@@ -654,7 +666,7 @@ static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Fi
 }
 
 // Fast multishift CG over a batch of nrhs right-hand sides (one shift list).
-// Blocks of kFastBlock iterations:
+// Blocks of kFastBlock64 (FP64) or kFastBlock32 (FP32) iterations:
 //   [ p = r + beta p ; t = Q p (|t|^2,|p|^2) ; r' = r - alpha (Q t + s0 p) (|r'|^2) ] x m
 //   then one deferred update of every shift's x_s, p_s (FP64) from the ring.
 // Krylov vectors are FP64 until every RHS has its base residual below
@@ -666,7 +678,7 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   SolverState h;
   init_cgm_state(h, nshift, shifts, tol_rel, max_iter);
   h.phase = PH_KP;
-  h.mblk = kFastBlock;
+  h.mblk = kFastBlock64;
   h.switch_rr = switch_tol * switch_tol;
   std::vector<SolverState> hs(nrhs, h);
   std::vector<Field> hx((size_t)nrhs * nshift), hp((size_t)nrhs * nshift), hb(nrhs);
@@ -686,10 +698,10 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   LYM_HIP(hipMemcpyAsync(P.bs_dev, hb.data(), sizeof(Field) * nrhs, hipMemcpyHostToDevice, st));
   const int gr = red_grid_flat(n);
   const int gf = flat_grid(n);
-  const int nslot = kFastBlock + 1;
-  const int64_t ring_stride = (int64_t)nslot * n;
+  const int nslot64 = kFastBlock64 + 1, nslot32 = kFastBlock32 + 1;
+  const int64_t ring_stride64 = (int64_t)nslot64 * n, ring_stride32 = (int64_t)nslot32 * n;
   hipLaunchKernelGGL(k_fast_init, dim3(gr, nrhs), dim3(kBlock), 0, st, n, P.bs_dev, (double2*)P.ring64.data,
-                     ring_stride, (double2*)P.plive64.data, P.xs_dev, P.ps_dev, nshift, P.partials);
+                     ring_stride64, (double2*)P.plive64.data, P.xs_dev, P.ps_dev, nshift, P.partials);
   LYM_HIP(hipGetLastError());
   int rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
   if (rc) return rc;
@@ -700,7 +712,10 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
     const Field pl = prec == P64 ? P.plive64 : P.plive32;
     const Field t = prec == P64 ? P.tb64 : P.tb32;
     const int gw = wilson_blocks(d, prec);  // Wilson partials per RHS at this precision
-    for (int j = 0; j < kFastBlock; ++j) {
+    const int mb = prec == P64 ? kFastBlock64 : kFastBlock32;
+    const int nslot = prec == P64 ? nslot64 : nslot32;
+    const int64_t ring_stride = prec == P64 ? ring_stride64 : ring_stride32;
+    for (int j = 0; j < mb; ++j) {
       if (prec == P64)
         hipLaunchKernelGGL(k_fast_kp<P64>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const double2*)ring.data,
                            ring_stride, nslot, (double2*)pl.data);
@@ -728,10 +743,10 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
       if (e) return e;
     }
     if (prec == P64)
-      hipLaunchKernelGGL((k_fast_k3<P64, kFastBlock>), dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
+      hipLaunchKernelGGL((k_fast_k3<P64, kFastBlock64>), dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
                          (const double2*)ring.data, ring_stride, nslot, P.xs_dev, P.ps_dev, nshift);
     else
-      hipLaunchKernelGGL((k_fast_k3<P32, kFastBlock>), dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
+      hipLaunchKernelGGL((k_fast_k3<P32, kFastBlock32>), dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
                          (const float2*)ring.data, ring_stride, nslot, P.xs_dev, P.ps_dev, nshift);
     LYM_HIP(hipGetLastError());
     return launch_fin(FK_FAST_K3ADV, P.st, nullptr, 0, 0, nullptr, st, nrhs, P.ctl);
@@ -739,7 +754,7 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   // poll the batch control block: (ndone, nswitch, nrhs, -)
   int rce = ensure_sync();
   if (rce) return rce;
-  const int64_t max_chunks = (int64_t)max_iter / kFastBlock + 8;
+  const int64_t max_chunks = (int64_t)max_iter / kFastBlock64 + 8;
   int64_t issued = 0, polled = 0;
   auto issue = [&](int slot) -> int {
     const int e = chunk();
@@ -763,8 +778,9 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
     }
     if (s4[1] >= nrhs && prec == P64 && lowprec == P32) {
       hipLaunchKernelGGL(k_fast_switch<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
-                         (const double2*)P.ring64.data, (float2*)P.ring32.data, ring_stride, nslot,
-                         (const double2*)P.plive64.data, (float2*)P.plive32.data);
+                         (const double2*)P.ring64.data, ring_stride64, nslot64, (float2*)P.ring32.data,
+                         ring_stride32, nslot32, (const double2*)P.plive64.data, (float2*)P.plive32.data,
+                         kFastBlock32);
       LYM_HIP(hipGetLastError());
       prec = P32;
     }

@@ -351,8 +351,9 @@ struct TileView {
   }
 };
 
-template <int NC, bool ADJ, int P, class Epi>
-__global__ __launch_bounds__(kTileMaxThreads) void k_wilson_tile(Lat lat, TileMap tm, SpinorView<P> in,
+template <int NC, bool ADJ, int P, class Epi, int HIF>
+__global__ __launch_bounds__(P == P64 ? kTileMaxThreads64 : (HIF == 1 ? kTileMaxThreads : 512),
+                             HIF == 1 ? 3 : 2) void k_wilson_tile(Lat lat, TileMap tm, SpinorView<P> in,
                                                                  int64_t in_stride,
                                                                  LinkView<typename Store<P>::R> lk,
                                                                  LinkView<typename Store<P>::R> lkr, Epi epi,
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(kTileMaxThreads) void k_wilson_tile(Lat lat, TileMa
 #pragma unroll
   for (int i = 0; i < NR; ++i) red[i] = 0.0;
   cpx<R> hop[NCOMP];
-  wilson_hop_x<4, NC, ADJ, true, R>(lat, s, x, view, lk, lkr, hop);
+  wilson_hop_x<4, NC, ADJ, true, R, TileView<P>, HIF>(lat, s, x, view, lk, lkr, hop);
   e.apply(lat, s, view, hop, red);
   if constexpr (Epi::NRED > 0)
     block_partials<NR, kTileMaxThreads / 64>(red, rc.partials + (int64_t)rhs * NR * tm.blocks, tblk, tm.blocks);
@@ -456,8 +457,21 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
       const TileMap& tm = d.tile[P == P64 ? 0 : 1];
       const size_t lds = (size_t)NCOMP * tm.nsite * sizeof(typename Store<P>::elem);
       const dim3 grid((unsigned)tm.blocks * (unsigned)nrhs), block(tm.nsite);
-      hipLaunchKernelGGL((k_wilson_tile<NC, ADJ, P, Epi>), grid, block, lds, stream, d.lat, tm, sv, a.rhs_stride, lk,
-                         lkr, epi, rc, nrhs);
+      // hops in flight: 4 is built for complex64 only (complex128 would
+      // spill); more than one needs blocks of at most 512 threads
+      int hif = (P == P64 && tm.hif == 4) ? 2 : tm.hif;
+      if (tm.nsite > 512) hif = 1;
+      if (hif == 4) {
+        if constexpr (P == P32)
+          hipLaunchKernelGGL((k_wilson_tile<NC, ADJ, P, Epi, 4>), grid, block, lds, stream, d.lat, tm, sv,
+                             a.rhs_stride, lk, lkr, epi, rc, nrhs);
+      } else if (hif == 2) {
+        hipLaunchKernelGGL((k_wilson_tile<NC, ADJ, P, Epi, 2>), grid, block, lds, stream, d.lat, tm, sv, a.rhs_stride,
+                           lk, lkr, epi, rc, nrhs);
+      } else {
+        hipLaunchKernelGGL((k_wilson_tile<NC, ADJ, P, Epi, 1>), grid, block, lds, stream, d.lat, tm, sv, a.rhs_stride,
+                           lk, lkr, epi, rc, nrhs);
+      }
       LYM_HIP(hipGetLastError());
       return LYM_OK;
     }

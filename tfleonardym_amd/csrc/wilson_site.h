@@ -134,97 +134,121 @@ __host__ __device__ __forceinline__ void transport(const cpx<R> (&u)[NC][NC], co
 //   PROJ = true : half-spinor projection + reconstruction (dirac.py:458-501)
 //   PROJ = false: full projector on all Ns rows (dirac.py:503-523)
 // SV: spinor view with load<NS*NCC>(site, v); links/links_r: LinkView<R>.
+// One hop (mu, dir: 0 = +mu, 1 = -mu) of a site with coordinate xmu along
+// mu: neighbour spinor at site nb, link at site ls (forward: U_mu(x);
+// backward: U_mu(x - mu)), accumulated into acc.
 template <int D, int NC, bool ADJ, bool PROJ, typename R, typename SV>
-__host__ __device__ __forceinline__ void wilson_hop_x(const Lat& lat, int64_t s, const int (&x)[D], const SV& psi,
-                                                      const LinkView<R>& links, const LinkView<R>& links_r,
-                                                      cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
+__host__ __device__ __forceinline__ void wilson_hop_one(const Lat& lat, int mu, int dir, int xmu, int64_t nb,
+                                                        int64_t ls, const SV& psi, const LinkView<R>& links,
+                                                        const LinkView<R>& links_r,
+                                                        cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
   constexpr int NS = Gam<D>::NS;
   constexpr int H = NS / 2;
   constexpr int NCC = ADJ ? NC * NC : NC;
   constexpr int NCOMP = NS * NCC;
+  const bool fwd = (dir == 0);
+  cpx<R> v[NCOMP];
+  psi.template load_hop<NCOMP>(mu, dir, nb, v);
+  cpx<R> u[NC][NC];
+  links.template load<NC>(mu, ls, u);
+  cpx<R> ur[NC][NC];
+  if constexpr (ADJ) links_r.template load<NC>(mu, ls, ur);
+  // forward term uses (1 - g), backward (1 + g): sg = -1 / +1
+  const R sg = fwd ? R(-1) : R(1);
+  const bool edge = lat.has_bc && (mu == D - 1) && (fwd ? xmu == lat.L[mu] - 1 : xmu == 0);
+  const cpx<R> bcf = fwd ? mk<R>((R)lat.bc_re, (R)lat.bc_im) : mk<R>((R)lat.bc_re, -(R)lat.bc_im);
+  if constexpr (PROJ) {
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      const int pr = Gam<D>::perm(mu, r);
+      const int kr = Gam<D>::k(mu, r);
+      cpx<R> h[NCC];
+#pragma unroll
+      for (int c = 0; c < NCC; ++c) {
+        const cpx<R> w = ipow_rt<R>(kr, v[(H + pr) * NCC + c]);
+        h[c] = mk<R>(v[r * NCC + c].re + sg * w.re, v[r * NCC + c].im + sg * w.im);
+      }
+      cpx<R> chi[NCC];
+      if (fwd) transport<NC, ADJ, true, R>(u, ur, h, chi);
+      else transport<NC, ADJ, false, R>(u, ur, h, chi);
+      if (edge) {
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) chi[c] = cmul(bcf, chi[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < NCC; ++c) {
+        acc[r * NCC + c] += chi[c];
+        const cpx<R> w = ipow_rt<R>(4 - kr, chi[c]);
+        acc[(H + pr) * NCC + c].re += sg * w.re;
+        acc[(H + pr) * NCC + c].im += sg * w.im;
+      }
+    }
+  } else {
+    // full projector rows: upper r and lower H+perm_r
+    cpx<R> full[NCOMP];
+#pragma unroll
+    for (int r = 0; r < H; ++r) {
+      const int pr = Gam<D>::perm(mu, r);
+      const int kr = Gam<D>::k(mu, r);
+#pragma unroll
+      for (int c = 0; c < NCC; ++c) {
+        const cpx<R> up = v[r * NCC + c];
+        const cpx<R> dn = v[(H + pr) * NCC + c];
+        const cpx<R> a = ipow_rt<R>(kr, dn);      // A psi_dn
+        const cpx<R> b = ipow_rt<R>(4 - kr, up);  // A^dag psi_up
+        full[r * NCC + c] = mk<R>(up.re + sg * a.re, up.im + sg * a.im);
+        full[(H + pr) * NCC + c] = mk<R>(dn.re + sg * b.re, dn.im + sg * b.im);
+      }
+    }
+#pragma unroll
+    for (int sp = 0; sp < NS; ++sp) {
+      cpx<R> chi[NCC];
+      if (fwd) transport<NC, ADJ, true, R>(u, ur, &full[sp * NCC], chi);
+      else transport<NC, ADJ, false, R>(u, ur, &full[sp * NCC], chi);
+      if (edge) {
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) chi[c] = cmul(bcf, chi[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < NCC; ++c) acc[sp * NCC + c] += chi[c];
+    }
+  }
+}
+
+// Hops in the order (mu, +), (mu, -), mu = 0..D-1, in groups of HIF: the
+// site offsets of a group pass through hop_dep after the previous group's
+// accumulators are committed, so HIF hops' loads are in flight per wave
+// (HIF = 1: one hop at a time, fewest registers; larger HIF: more memory
+// parallelism for lattices too small to fill the chip).  The accumulation
+// order, hence the result, does not depend on HIF.
+template <int D, int NC, bool ADJ, bool PROJ, typename R, typename SV, int HIF = 1>
+__host__ __device__ __forceinline__ void wilson_hop_x(const Lat& lat, int64_t s, const int (&x)[D], const SV& psi,
+                                                      const LinkView<R>& links, const LinkView<R>& links_r,
+                                                      cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
+  constexpr int NCOMP = Gam<D>::NS * (ADJ ? NC * NC : NC);
+  static_assert((2 * D) % HIF == 0, "hops per group must divide 2D");
 #pragma unroll
   for (int c = 0; c < NCOMP; ++c) acc[c] = czero<R>();
-
 #pragma unroll
-  for (int mu = 0; mu < D; ++mu) {
-    const int L = lat.L[mu];
-    const int64_t st = lat.stride[mu];
-    const bool bc_edge_f = lat.has_bc && (mu == D - 1) && (x[mu] == L - 1);
-    const bool bc_edge_b = lat.has_bc && (mu == D - 1) && (x[mu] == 0);
+  for (int g = 0; g < 2 * D; g += HIF) {
+    int64_t nb[HIF], ls[HIF];
 #pragma unroll
-    for (int dir = 0; dir < 2; ++dir) {
-      const bool fwd = (dir == 0);
-      int64_t nb = fwd ? ((x[mu] == L - 1) ? s - (int64_t)(L - 1) * st : s + st)
-                       : ((x[mu] == 0) ? s + (int64_t)(L - 1) * st : s - st);
-      int64_t ls = fwd ? s : nb;  // forward: U_mu(x); backward: U_mu(x-mu)
-      hop_commit<NCOMP, R>(acc);
-      hop_dep<R>(nb, ls, acc[0].re);
-      cpx<R> v[NCOMP];
-      psi.template load_hop<NCOMP>(mu, dir, nb, v);
-      cpx<R> u[NC][NC];
-      links.template load<NC>(mu, ls, u);
-      cpx<R> ur[NC][NC];
-      if constexpr (ADJ) links_r.template load<NC>(mu, ls, ur);
-      // forward term uses (1 - g), backward (1 + g): sg = -1 / +1
-      const R sg = fwd ? R(-1) : R(1);
-      const bool edge = fwd ? bc_edge_f : bc_edge_b;
-      const cpx<R> bcf = fwd ? mk<R>((R)lat.bc_re, (R)lat.bc_im) : mk<R>((R)lat.bc_re, -(R)lat.bc_im);
-      if constexpr (PROJ) {
+    for (int q = 0; q < HIF; ++q) {
+      const int mu = (g + q) >> 1;
+      const bool fwd = ((g + q) & 1) == 0;
+      const int L = lat.L[mu];
+      const int64_t st = lat.stride[mu];
+      nb[q] = fwd ? ((x[mu] == L - 1) ? s - (int64_t)(L - 1) * st : s + st)
+                  : ((x[mu] == 0) ? s + (int64_t)(L - 1) * st : s - st);
+      ls[q] = fwd ? s : nb[q];
+    }
+    hop_commit<NCOMP, R>(acc);
 #pragma unroll
-        for (int r = 0; r < H; ++r) {
-          const int pr = Gam<D>::perm(mu, r);
-          const int kr = Gam<D>::k(mu, r);
-          cpx<R> h[NCC];
+    for (int q = 0; q < HIF; ++q) hop_dep<R>(nb[q], ls[q], acc[0].re);
 #pragma unroll
-          for (int c = 0; c < NCC; ++c) {
-            const cpx<R> w = ipow_rt<R>(kr, v[(H + pr) * NCC + c]);
-            h[c] = mk<R>(v[r * NCC + c].re + sg * w.re, v[r * NCC + c].im + sg * w.im);
-          }
-          cpx<R> chi[NCC];
-          if (fwd) transport<NC, ADJ, true, R>(u, ur, h, chi);
-          else transport<NC, ADJ, false, R>(u, ur, h, chi);
-          if (edge) {
-#pragma unroll
-            for (int c = 0; c < NCC; ++c) chi[c] = cmul(bcf, chi[c]);
-          }
-#pragma unroll
-          for (int c = 0; c < NCC; ++c) {
-            acc[r * NCC + c] += chi[c];
-            const cpx<R> w = ipow_rt<R>(4 - kr, chi[c]);
-            acc[(H + pr) * NCC + c].re += sg * w.re;
-            acc[(H + pr) * NCC + c].im += sg * w.im;
-          }
-        }
-      } else {
-        // full projector rows: upper r and lower H+perm_r
-        cpx<R> full[NCOMP];
-#pragma unroll
-        for (int r = 0; r < H; ++r) {
-          const int pr = Gam<D>::perm(mu, r);
-          const int kr = Gam<D>::k(mu, r);
-#pragma unroll
-          for (int c = 0; c < NCC; ++c) {
-            const cpx<R> up = v[r * NCC + c];
-            const cpx<R> dn = v[(H + pr) * NCC + c];
-            const cpx<R> a = ipow_rt<R>(kr, dn);        // A psi_dn
-            const cpx<R> b = ipow_rt<R>(4 - kr, up);    // A^dag psi_up
-            full[r * NCC + c] = mk<R>(up.re + sg * a.re, up.im + sg * a.im);
-            full[(H + pr) * NCC + c] = mk<R>(dn.re + sg * b.re, dn.im + sg * b.im);
-          }
-        }
-#pragma unroll
-        for (int sp = 0; sp < NS; ++sp) {
-          cpx<R> chi[NCC];
-          if (fwd) transport<NC, ADJ, true, R>(u, ur, &full[sp * NCC], chi);
-          else transport<NC, ADJ, false, R>(u, ur, &full[sp * NCC], chi);
-          if (edge) {
-#pragma unroll
-            for (int c = 0; c < NCC; ++c) chi[c] = cmul(bcf, chi[c]);
-          }
-#pragma unroll
-          for (int c = 0; c < NCC; ++c) acc[sp * NCC + c] += chi[c];
-        }
-      }
+    for (int q = 0; q < HIF; ++q) {
+      const int mu = (g + q) >> 1;
+      wilson_hop_one<D, NC, ADJ, PROJ, R, SV>(lat, mu, (g + q) & 1, x[mu], nb[q], ls[q], psi, links, links_r, acc);
     }
   }
 }
