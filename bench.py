@@ -174,11 +174,14 @@ def run_batched(args):
     print(json.dumps(out))
 
 
-def bytes_per_site_dslash(nc, ndim, wl, ws):
-    """Compulsory bytes of one D application per site: links read once,
-    input and output spinor once (w = bytes per real)."""
+def bytes_per_site_dslash(nc, ndim, wl, ws, link_rows=None):
+    """Compulsory bytes of one D application per site: links read once
+    (link_rows of nc complex each: nc, or nc - 1 when the kernels rebuild the
+    last row of SU(nc) links), input and output spinor once (w = bytes per
+    real)."""
     ns = 4 if ndim == 4 else 2
-    return ndim * nc * nc * 2 * wl + 2 * ns * nc * 2 * ws
+    rows = nc if link_rows is None else link_rows
+    return ndim * rows * nc * 2 * wl + 2 * ns * nc * 2 * ws
 
 
 def main():
@@ -284,7 +287,7 @@ def main():
         e1.record(stream)
         e1.synchronize()
         us = e0.elapsed_time(e1) * 1e3 / args.dslash_reps
-        bps = bytes_per_site_dslash(nc, 4, wl, wl)
+        bps = bytes_per_site_dslash(nc, 4, wl, wl, nc - 1 if desc.link_format == _lib.LINKS_SU else nc)
         res[name] = dict(us=us, bytes_per_site=bps, gbs=bps * V / (us * 1e-6) / 1e9)
         print(f"[bench] wilson {name}: {us:.2f} us/launch, {res[name]['gbs']:.0f} GB/s", file=sys.stderr,
               flush=True)
@@ -328,7 +331,9 @@ def main():
         "speedup_vs_plain_fp64": (solves_per_s / plain) if plain else None,
         "max_true_rel_resid": max(stats["true_rel_resid"]),
         "dslash_gbs": {"f64": res["f64"]["gbs"], "f32": f32["gbs"]},
-        "roofline": {"kernel": "k_wilson f32 (D, one RHS)", "bound": "hbm", "achieved": f32["gbs"],
+        "roofline": {"kernel": "Wilson D complex64, one RHS (k_wilson_tile)", "bound": "hbm",
+                     "achieved": f32["gbs"], "link_format": "su-rebuilt" if desc.link_format == _lib.LINKS_SU
+                     else "full",
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": f32["gbs"] / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_site": f32["bytes_per_site"], "sites": V, "avg_us": f32["us"]},
         "cpu_baseline": cpu,

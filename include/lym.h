@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define LYM_ABI_VERSION 1
+#define LYM_ABI_VERSION 2
 
 enum {
   LYM_OK = 0,
@@ -61,7 +61,17 @@ typedef struct lym_wilson {
   const void* links_right;      /* complex128 or NULL (adjoint only) */
   const void* links_f32;        /* complex64 copy of links, or NULL */
   const void* links_right_f32;  /* complex64 copy of links_right, or NULL */
+  /* LYM_LINKS_SU: every link is special unitary (checked by the caller with
+   * lym_link_deviation); fundamental-representation kernels then read the
+   * first nc-1 rows of each link and rebuild the last one (SU(3): row 2 =
+   * conj(row 0 x row 1); SU(2): row 1 = (-conj(u01), conj(u00))).
+   * LYM_LINKS_FULL: every entry is read (any complex matrices). */
+  int32_t link_format;
+  int32_t reserved;
 } lym_wilson;
+
+#define LYM_LINKS_FULL 0
+#define LYM_LINKS_SU 1
 
 typedef struct lym_solve_stats {
   int32_t status;      /* 1 converged, 2 max iterations, 3 breakdown */
@@ -115,6 +125,12 @@ int lym_gamma5(int prec, int ns, int64_t n_per_spin, const void* in, void* out, 
 /* Precision conversion of n complex numbers (F64 <-> F32). No reference
  * counterpart: the reference runs in complex128 only. */
 int lym_convert(int prec_in, const void* in, int prec_out, void* out, int64_t n, void* stream);
+/* dev_out[0] (device double) = max over links and entries of
+ * |stored last row - row rebuilt from the others| for op->links (F64) or
+ * op->links_f32 (F32), the test that allows LYM_LINKS_SU.  No reference
+ * counterpart (the reference always reads the full matrix); the boundary
+ * conditions materialised by dirac.py:351-378 leave links that fail it. */
+int lym_link_deviation(const lym_wilson* op, int prec, double* dev_out, void* stream);
 
 /* ---- solvers ---------------------------------------------------------- */
 /* Device workspace (bytes) for the solvers below. */

@@ -41,13 +41,15 @@ def test_python_binding_covers_header(lib):
 
 
 def test_abi_version(lib):
+    from tfleonardym_amd import _lib
+    want = int(re.search(r"#define LYM_ABI_VERSION (\d+)", open(HEADER).read()).group(1))
     lib.lym_abi_version.restype = ctypes.c_int
-    assert lib.lym_abi_version() == 1
+    assert lib.lym_abi_version() == want == _lib.ABI_VERSION
 
 
 def test_struct_layout_matches_header():
     """ctypes mirror of lym_wilson / lym_solve_stats has the C sizes."""
     from tfleonardym_amd import _lib
-    # 4 int32 + 4 int32 dims + 3 double + 4 pointers
-    assert ctypes.sizeof(_lib.LymWilson) == 4 * 4 + 4 * 4 + 3 * 8 + 4 * 8
+    # 4 int32 + 4 int32 dims + 3 double + 4 pointers + link_format + reserved
+    assert ctypes.sizeof(_lib.LymWilson) == 4 * 4 + 4 * 4 + 3 * 8 + 4 * 8 + 2 * 4
     assert ctypes.sizeof(_lib.LymSolveStats) == 4 * 4 + 8 + 8 + 64 * 4 + 64 * 8

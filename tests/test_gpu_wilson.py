@@ -127,6 +127,39 @@ def test_lds_tile_shapes(prec, geometry, monkeypatch):
         monkeypatch.delenv("LYM_HIF")
 
 
+@pytest.mark.parametrize("nc", [2, 3])
+def test_su_link_rebuild(nc, monkeypatch):
+    """SU(N) links are flagged LINKS_SU and the kernels rebuild each link's
+    last row: D matches the oracle (which reads all entries) and the
+    full-read kernels (LYM_FULL_LINKS=1) to FP64 rounding.  Links that are not
+    SU(N) (random complex matrices; -U for SU(3), det = -1) are flagged
+    LINKS_FULL and still match the oracle."""
+    from tfleonardym_amd import _lib
+    print(f"case su link rebuild nc={nc}", flush=True)
+    geometry = (8, 4, 4, 8)
+    U = fields.gauge_field(geometry, nc, seed=51, kind="haar")
+    psi = fields.gaussian_spinor(geometry, 4, nc, seed=52)
+    want = ow.WilsonOracle(U, geometry, 0.13, True, "fundamental", bc_factor=-1.0).apply(psi)
+    op = make_op(U, geometry, 0.13, True, "fundamental", -1.0)
+    assert op.desc.link_format == _lib.LINKS_SU
+    got = op.apply(psi)
+    assert rel(got, want) < TOL64
+    monkeypatch.setenv("LYM_FULL_LINKS", "1")
+    op_full = make_op(U, geometry, 0.13, True, "fundamental", -1.0)
+    assert op_full.desc.link_format == _lib.LINKS_FULL
+    assert rel(got, op_full.apply(psi).cpu().numpy()) < 1e-14
+    monkeypatch.delenv("LYM_FULL_LINKS")
+    rng = np.random.default_rng(5)
+    bad = [rng.standard_normal(U.shape) + 1j * rng.standard_normal(U.shape)]
+    if nc == 3:
+        bad.append(-U)
+    for W in bad:
+        op_w = make_op(W, geometry, 0.13, True, "fundamental", -1.0)
+        assert op_w.desc.link_format == _lib.LINKS_FULL
+        want_w = ow.WilsonOracle(W, geometry, 0.13, True, "fundamental", bc_factor=-1.0).apply(psi)
+        assert rel(op_w.apply(psi), want_w) < TOL64
+
+
 def test_custom_provider_is_materialised():
     """A provider that is not the standard one is applied once to the links."""
     from tfleonardym_amd import boundary_conditions, dirac, lattice

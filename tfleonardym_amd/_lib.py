@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "liblym.so")
 PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 REP_FUNDAMENTAL, REP_ADJOINT = 0, 1
 MODE_PROJECTED, MODE_SIMPLE = 0, 1
+LINKS_FULL, LINKS_SU = 0, 1
+ABI_VERSION = 2  # LYM_ABI_VERSION of include/lym.h
 
 OK = 0
 NOT_CONVERGED = 1
@@ -35,6 +37,8 @@ class LymWilson(ctypes.Structure):
         ("links_right", ctypes.c_void_p),
         ("links_f32", ctypes.c_void_p),
         ("links_right_f32", ctypes.c_void_p),
+        ("link_format", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -81,6 +85,7 @@ SIGNATURES = {
     "lym_axpby": (_i32, [_i32, _i64, _f64, _f64, _vp, _f64, _f64, _vp, _vp]),
     "lym_gamma5": (_i32, [_i32, _i32, _i64, _vp, _vp, _vp]),
     "lym_convert": (_i32, [_i32, _vp, _i32, _vp, _i64, _vp]),
+    "lym_link_deviation": (_i32, [_vp, _i32, _vp, _vp]),
     "lym_solver_workspace_bytes": (_sz, [ctypes.POINTER(LymWilson), _i32, _i32]),
     "lym_mr_multishift": (_i32, [ctypes.POINTER(LymWilson), _i32, ctypes.POINTER(_f64), _i32, _vp,
                                  ctypes.POINTER(_vp), _i32, _f64, _f64, _vp, _sz,
@@ -110,7 +115,7 @@ def load(path=LIB_PATH):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.lym_abi_version() != 1:
+    if lib.lym_abi_version() != ABI_VERSION:
         raise ImportError("liblym.so ABI version mismatch; rebuild")
     _lib = lib
     return lib
@@ -140,6 +145,30 @@ def stream_handle(device=None):
 
 def ptr(t):
     return ctypes.c_void_p(t.data_ptr())
+
+
+# Largest |stored - rebuilt| link entry for which the kernels may rebuild the
+# last row of every link (LINKS_SU).  Unitarised FP64 configurations sit near
+# 1e-15; at 1e-14 the rebuilt rows change D psi by well under 1e-14 relative.
+SU_TOLERANCE = 1e-14
+
+
+def su_link_format(desc, device=None):
+    """LINKS_SU if every complex128 link of `desc` is special unitary to
+    SU_TOLERANCE (measured on the device by lym_link_deviation), else
+    LINKS_FULL.  LYM_FULL_LINKS=1 in the environment forces LINKS_FULL."""
+    if os.environ.get("LYM_FULL_LINKS") == "1":
+        return LINKS_FULL
+    dev = device if device is not None else require_device()
+    out = torch.empty(1, dtype=torch.float64, device=dev)
+    saved = desc.link_format
+    desc.link_format = LINKS_FULL
+    try:
+        check(load().lym_link_deviation(ctypes.byref(desc), PREC_F64, ptr(out), stream_handle(dev)),
+              "link deviation")
+    finally:
+        desc.link_format = saved
+    return LINKS_SU if float(out.item()) <= SU_TOLERANCE else LINKS_FULL
 
 
 _scratch = {}

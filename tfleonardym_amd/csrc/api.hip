@@ -76,6 +76,11 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   o.linksr64 = w->links_right;
   o.links32 = w->links_f32;
   o.linksr32 = w->links_right_f32;
+  if (w->link_format != LYM_LINKS_FULL && w->link_format != LYM_LINKS_SU) {
+    set_error("unknown link format");
+    return LYM_ERR_ARG;
+  }
+  o.link_su = w->link_format == LYM_LINKS_SU && !o.adj;
   // block-resident field bytes must stay below 4 GiB (32-bit buffer offsets)
   const int64_t link_bytes = (int64_t)w->ndim * w->nc * w->nc * V * 16;
   const int64_t field_bytes = (int64_t)o.ncomp * V * 16;
@@ -267,6 +272,17 @@ int lym_axpby(int prec, int64_t n, double a_re, double a_im, const void* x, doub
   }
   if (n == 0) return LYM_OK;
   return blas_axpby(prec, n, a_re, a_im, x, b_re, b_im, y, (hipStream_t)stream);
+}
+
+int lym_link_deviation(const lym_wilson* op, int prec, double* dev_out, void* stream) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (!dev_out || (prec != LYM_PREC_F64 && prec != LYM_PREC_F32)) {
+    set_error("lym_link_deviation: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  return link_deviation(d, prec, dev_out, (hipStream_t)stream);
 }
 
 int lym_gamma5(int prec, int ns, int64_t n_per_spin, const void* in, void* out, void* stream) {

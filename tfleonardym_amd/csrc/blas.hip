@@ -180,4 +180,70 @@ int blas_gamma5(int prec, int ns, int64_t nps, const void* in, void* out, hipStr
   return LYM_OK;
 }
 
+// ---- special-unitarity test of the links ----------------------------------
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// out = max over (mu, site) and entries of |stored last row - rebuilt last row|
+// (the rebuild LinkView uses with recon = 1).  The maximum of non-negative
+// doubles is an atomicMax on their bit patterns: exact and order independent.
+template <typename R, int NC>
+__global__ __launch_bounds__(kBlock) void k_link_dev(const cpx<R>* __restrict__ links, int64_t V, int ndim,
+                                                     unsigned long long* out) {
+  double m = 0.0;
+  const LinkView<R> full{links, V, 0};
+  const LinkView<R> su{links, V, 1};
+  const int64_t n = (int64_t)ndim * V;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int mu = (int)(i / V);
+    const int64_t s = i - (int64_t)mu * V;
+    cpx<R> a[NC][NC], b[NC][NC];
+    full.template load<NC>(mu, s, a);
+    su.template load<NC>(mu, s, b);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      const double dr = fabs((double)a[NC - 1][j].re - (double)b[NC - 1][j].re);
+      const double di = fabs((double)a[NC - 1][j].im - (double)b[NC - 1][j].im);
+      // a NaN difference (non-finite entries) fails the test
+      if (!(dr <= m)) m = (dr == dr) ? dr : INFINITY;
+      if (!(di <= m)) m = (di == di) ? di : INFINITY;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  __shared__ double sm[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kBlock / 64; ++w) m = fmax(m, sm[w]);
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+int link_deviation(const OpDesc& d, int prec, double* out_dev, hipStream_t st) {
+  const void* l = prec == P64 ? d.links64 : d.links32;
+  if (!l) {
+    set_error("lym_link_deviation: the operator has no links at this precision");
+    return LYM_ERR_ARG;
+  }
+  unsigned long long* o = reinterpret_cast<unsigned long long*>(out_dev);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, o);
+  const int g = red_grid((int64_t)d.ndim * d.lat.V);
+  if (prec == P64) {
+    if (d.nc == 3)
+      hipLaunchKernelGGL((k_link_dev<double, 3>), dim3(g), dim3(kBlock), 0, st, (const cpx<double>*)l, d.lat.V,
+                         d.ndim, o);
+    else
+      hipLaunchKernelGGL((k_link_dev<double, 2>), dim3(g), dim3(kBlock), 0, st, (const cpx<double>*)l, d.lat.V,
+                         d.ndim, o);
+  } else {
+    if (d.nc == 3)
+      hipLaunchKernelGGL((k_link_dev<float, 3>), dim3(g), dim3(kBlock), 0, st, (const cpx<float>*)l, d.lat.V,
+                         d.ndim, o);
+    else
+      hipLaunchKernelGGL((k_link_dev<float, 2>), dim3(g), dim3(kBlock), 0, st, (const cpx<float>*)l, d.lat.V,
+                         d.ndim, o);
+  }
+  LYM_HIP(hipGetLastError());
+  return LYM_OK;
+}
+
 }  // namespace lym

@@ -28,6 +28,7 @@ struct OpDesc {
   int wthreads = 0;  // plain Wilson block size (threads, multiple of 64)
   TileMap tile[2]{};  // LDS-tiled kernel per precision: [0] complex128, [1] complex64
   int use_tile = 1;   // 0 forces the plain kernel
+  int link_su = 0;    // links are SU(nc): fundamental kernels rebuild the last row
   const void* links64 = nullptr;
   const void* linksr64 = nullptr;
   const void* links32 = nullptr;
@@ -109,6 +110,8 @@ int blas_convert(int pin, const void* in, int pout, void* out, int64_t n, hipStr
 int blas_gamma5(int prec, int ns, int64_t nps, const void* in, void* out, hipStream_t st);
 // out_dev[0] = sum |a-b|^2, out_dev[1] = sum |a|^2
 int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* out_dev, RedCtl rc, hipStream_t st);
+// out_dev[0] = max |stored - rebuilt| over the last rows of every link
+int link_deviation(const OpDesc& d, int prec, double* out_dev, hipStream_t st);
 
 // Largest number of blocks any reducing kernel launches for volume V
 // (Wilson blocks have at least 64 threads).

@@ -254,17 +254,38 @@ struct SpinorView {
 };
 
 // Links: (D, Nc, Nc, V) complex, site fastest.  Link precision is P64 or P32.
+// recon = 1: the links are special unitary; only rows 0..NC-2 are read and
+// the last row is rebuilt (SU(3): u2 = conj(u0 x u1); SU(2): u1 = (-u01*, u00*)),
+// one third (SU(3)) or half (SU(2)) fewer link bytes per hop.
 template <typename R>
 struct LinkView {
   const cpx<R>* data;
   int64_t V;
+  int recon = 0;
   template <int NC>
   __host__ __device__ __forceinline__ void load(int mu, int64_t s, cpx<R> (&u)[NC][NC]) const {
     const uint32_t bytes = (uint32_t)((int64_t)(mu + 1) * NC * NC * V * (int64_t)sizeof(cpx<R>));
 #pragma unroll
-    for (int i = 0; i < NC; ++i)
+    for (int i = 0; i < NC - 1; ++i)
 #pragma unroll
       for (int j = 0; j < NC; ++j) u[i][j] = ld<cpx<R>>(data, bytes, (int64_t)(mu * NC + i) * NC + j, V, s);
+    if (recon) {
+      if constexpr (NC == 3) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const int a = (j + 1) % 3, b = (j + 2) % 3;
+          const cpx<R> p = cmul(u[0][a], u[1][b]);
+          const cpx<R> q = cmul(u[0][b], u[1][a]);
+          u[2][j] = mk<R>(p.re - q.re, q.im - p.im);
+        }
+      } else if constexpr (NC == 2) {
+        u[1][0] = mk<R>(-u[0][1].re, u[0][1].im);
+        u[1][1] = mk<R>(u[0][0].re, -u[0][0].im);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) u[NC - 1][j] = ld<cpx<R>>(data, bytes, (int64_t)(mu * NC + NC - 1) * NC + j, V, s);
+    }
   }
 };
 

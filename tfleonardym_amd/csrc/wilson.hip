@@ -448,8 +448,9 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
     set_error(P == P64 ? "operator has no complex128 links" : "operator has no complex64 links (call with a float32 copy)");
     return LYM_ERR_ARG;
   }
-  LinkView<R> lk{(const cpx<R>*)l, V};
-  LinkView<R> lkr{(const cpx<R>*)lr, V};
+  const int recon = (!ADJ && d.link_su) ? 1 : 0;
+  LinkView<R> lk{(const cpx<R>*)l, V, recon};
+  LinkView<R> lkr{(const cpx<R>*)lr, V, recon};
   SpinorView<P> sv{(typename Store<P>::elem*)in.data, in.norm, V};
   Epi epi(a, V);
   if constexpr (D == 4 && P != P16 && !ADJ) {

@@ -269,6 +269,9 @@ class DiracWilsonOperator(Operator):
         d.bc_im = float(self._bc.imag)
         d.links = links.data_ptr()
         d.links_right = links_right.data_ptr() if links_right is not None else None
+        # special-unitary links: the kernels read nc-1 rows and rebuild the last
+        if rep == _lib.REP_FUNDAMENTAL:
+            d.link_format = _lib.su_link_format(d, links.device)
         return d
 
     def ensure_f32(self):

@@ -94,6 +94,7 @@ class GaugeStore:
         d.bc_im = float(np.imag(bc))
         d.links = U.data_ptr()
         d.links_f32 = U32.data_ptr()
+        d.link_format = _lib.su_link_format(d, self.device)
         self._entries[key] = dict(U=U, U32=U32, desc=d, geometry=geometry)
         return key
 
