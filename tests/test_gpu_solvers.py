@@ -81,6 +81,43 @@ def test_batched_cgm(mode, geometry):
             assert rel(xn, x1.cpu().numpy()) < 1e-8
 
 
+@pytest.mark.parametrize("shift", [0.0, 0.05])
+def test_mixed_cg_reliable_updates(shift):
+    """One shift, mixed precision: complex64 CG with FP64 reliable updates.
+    Every source of a batch reaches 1e-10 (true residual recomputed by the
+    oracle), matches the FP64 oracle CG to 1e-8, did at least one reliable
+    update, and gives the same bits as its own single-source solve (sources
+    do not interact in a batch)."""
+    from tfleonardym_amd import dirac, dirac_solver
+    geometry = (8, 4, 4, 16)
+    print(f"case mixed reliable CG shift={shift}", flush=True)
+    op, oop = setup(geometry, 3, 0.13)
+    srcs = [fields.gaussian_spinor(geometry, 4, 3, seed=60),
+            fields.z2_noise(geometry, 4, 3, seed=61),
+            fields.point_source(geometry, 4, 3, site=5, color=2, spin=1)]
+    s = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="mixed",
+                                       verbose=False)
+    sols = s.solve_batch(dirac.SquareOperator(op), srcs, [shift])
+    assert s.last_status == "converged", s.last_stats
+    for b, row, stt in zip(srcs, sols, s.last_stats):
+        xn = row[0].cpu().numpy()
+        assert osv.true_residual(oop.apply_square, b, xn, shift) <= 1e-10
+        xr, _ = osv.cg(oop.apply_square, b, shift, tol=1e-12)
+        assert rel(xn, xr) < 1e-8
+        assert stt["restarts"] >= 1
+        single = s.solve(dirac.SquareOperator(op), b, [shift])
+        assert torch.equal(row[0], single[0])
+
+
+def test_mixed_cg_max_steps():
+    from tfleonardym_amd import dirac, dirac_solver
+    op, _ = setup((4, 4, 4, 4), 3, 0.13)
+    b = fields.gaussian_spinor((4, 4, 4, 4), 4, 3, seed=8)
+    s = dirac_solver.multishift_solver(3, 0, algorithm="cg", precision_mode="mixed", verbose=False)
+    s.solve(dirac.SquareOperator(op), b, [0.01])
+    assert s.last_status == "max_iterations"
+
+
 def test_c1_plain_fp64_cg():
     """The CPU-runnable workload: 2D 24x16 SU(2), Haar links, kappa=-0.15,
     periodic x / antiperiodic t, FP64 CG on D^dag D to 1e-10."""

@@ -138,6 +138,57 @@ __device__ inline void cgm_fin_b_fast(SolverState* st, double rrn) {
   st->phase = (st->final_k3 || st->blk_j >= st->mblk) ? PH_K3 : PH_KP;
 }
 
+// Mixed CG after the low-precision <r_{k+1}, r_{k+1}>: beta = rrn / rr and
+// p = r + beta p next (PH_KP), unless a reliable update is due: the iterated
+// residual fell by delta below its largest value since the last update, or
+// passed the stopping test, or the iteration budget is spent (PH_RU; rr keeps
+// <r_k, r_k> so the update can form beta from the true residual).
+__device__ inline void cg_fin_b_rel(SolverState* st, double rrn) {
+  st->iter += 1;
+  if (!isfinite(rrn)) {
+    st->status = ST_BREAKDOWN;
+    st->phase = PH_DONE;
+    return;
+  }
+  const bool stop = rrn <= st->tol2_s[0];
+  if (st->iter >= st->max_iter) st->status = ST_MAXITER;
+  if (stop || rrn <= st->delta2 * st->rr_max || st->status == ST_MAXITER) {
+    st->ru_final = stop ? 1 : 0;
+    st->phase = PH_RU;
+    return;
+  }
+  st->beta_old = rrn / st->rr;
+  st->rr = rrn;
+  st->phase = PH_KP;
+}
+
+// After the FP64 true residual of a reliable update: converged when it meets
+// the target, stopped at the iteration budget, else the low-precision
+// recurrence restarts from it with p = r_true + (rr_true / rr_k) p.
+__device__ inline void cg_fin_ru(SolverState* st, double rr_true) {
+  st->n_ru += 1;
+  if (!isfinite(rr_true)) {
+    st->status = ST_BREAKDOWN;
+    st->phase = PH_DONE;
+    return;
+  }
+  if (rr_true <= st->tol2_true) {
+    st->status = ST_CONVERGED;
+    st->rr = rr_true;
+    st->phase = PH_DONE;
+    return;
+  }
+  if (st->status == ST_MAXITER) {
+    st->rr = rr_true;
+    st->phase = PH_DONE;
+    return;
+  }
+  st->beta_old = rr_true / st->rr;
+  st->rr = rr_true;
+  st->rr_max = rr_true;
+  st->phase = PH_KP;
+}
+
 // Multishift MR (dirac_solver.py:35-43): alpha = omega <p,r>/<p,p>;
 // f_i <- f_i / (1 + (s_i - s_N) alpha); x update coefficients c_i = alpha f_i.
 __device__ inline void mr_fin(SolverState* st, double pr_re, double pr_im, double pp) {

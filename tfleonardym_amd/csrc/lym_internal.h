@@ -50,6 +50,7 @@ struct EpiArgs {
   // rings by i * ring_rhs_stride, and SolverState st + i
   int64_t rhs_stride = 0;
   int64_t ring_rhs_stride = 0;
+  const Field* bs = nullptr;  // device array of per-RHS sources (reliable update)
 };
 
 // error reporting (thread local message)
@@ -73,6 +74,9 @@ enum EpiKind : int {
   EPI_MR_K1 = 4,     // out = Op in (phase gated)
   EPI_MR_K2 = 5,     // out = Op in + sigma aux0; reduce <out,aux0>, <out,out>; MR finalizer
   EPI_CGM_K2F = 6,   // fast CG-M: ring[k+1] = ring[k] - alpha (Op in + sigma0 aux0); deferred finalizer
+  EPI_CG_K2X = 7,    // mixed CG: aux1 -= alpha (Op in + sigma0 aux0); out += alpha aux0; reduce |aux1|^2
+  EPI_RU1 = 8,       // reliable update (complex128, gate RU): out = Op in
+  EPI_RU2 = 9,       // reliable update: r = bs - (Op in + sigma0 aux0) -> out (complex64); reduce |r|^2
 };
 int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, const EpiArgs& a, RedCtl rc,
                   hipStream_t stream, int nrhs = 1);
@@ -96,6 +100,9 @@ enum FinKind : int {
   FK_FAST_K3ADV = 6,  // gate K3: clear block, phase -> KP or DONE
   FK_MR = 7,          // gate K1: MR alpha and shift coefficients
   FK_MR_RR = 8,       // gate K3: MR stop test
+  FK_CG_A = 9,        // gate K1|KP: mixed CG step length (counts a breakdown as done)
+  FK_CG_B = 10,       // gate K2: mixed CG beta / reliable-update decision
+  FK_CG_RU = 11,      // gate RU: mixed CG after the FP64 true residual
 };
 // nb finalizer blocks: block i handles state st + i, partials + i * nr * G, out + i * nr
 int launch_finalize(FinKind kind, SolverState* st, const double* partials, int G, int nr, double* out,

@@ -17,6 +17,7 @@ enum Phase : int {
   PH_K3 = 2,    // per-shift vector updates (plain) / deferred block flush (fast)
   PH_DONE = 3,
   PH_KP = 4,    // fast path: p = r + beta p
+  PH_RU = 5,    // mixed CG: reliable update (FP64 x += x_lo, true residual)
 };
 
 enum SolveStatus : int { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXITER = 2, ST_BREAKDOWN = 3 };
@@ -62,6 +63,15 @@ struct SolverState {
   double h_alpha[kMaxBlock][kMaxShifts];
   double h_zeta[kMaxBlock][kMaxShifts];
   double h_beta[kMaxBlock][kMaxShifts];
+  // ---- mixed-precision CG with reliable updates (one shift): the low-
+  // precision recurrence runs until rr < delta2 * rr_max (rr_max = largest
+  // rr since the last update), then x_hi += x_lo and r = b - (A + s) x_hi in
+  // FP64 replace the iterated residual.
+  double rr_max;
+  double delta2;
+  double tol2_true;  // target on the true rr (absolute after the init scaling)
+  int ru_final;  // the pending update follows the iterated stopping test
+  int n_ru;      // reliable updates done
 };
 
 }  // namespace lym

@@ -88,6 +88,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       case FK_FAST_K3ADV: run = (ph == PH_K3); break;
       case FK_MR: run = (ph == PH_K1); break;
       case FK_MR_RR: run = (ph == PH_K3); break;
+      case FK_CG_A: run = (ph == PH_KP); break;
+      case FK_CG_B: run = (ph == PH_K2); break;
+      case FK_CG_RU: run = (ph == PH_RU); break;
       default: run = true;
     }
   }
@@ -119,9 +122,16 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       st->rr = sums[0];
       for (int k = 0; k < st->nshift; ++k) st->tol2_s[k] *= sums[0];
       st->switch_rr *= sums[0];
+      st->tol2_true *= sums[0];
+      st->rr_max = sums[0];
       if (!(sums[0] > 0.0)) {
         st->status = ST_CONVERGED;
         st->phase = PH_DONE;
+        if (ctl) {  // a zero source is finished (and needs no precision switch)
+          atomicAdd(&ctl->ndone, 1);
+          st->switch_counted = 1;
+          atomicAdd(&ctl->nswitch, 1);
+        }
       }
       break;
     case FK_CGM_A: cgm_fin_a(st, sums[0], sums[1]); break;
@@ -148,6 +158,14 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       break;
     case FK_MR: mr_fin(st, sums[0], sums[1], sums[2]); break;
     case FK_MR_RR: mr_fin_rr(st, sums[0]); break;
+    case FK_CG_A:
+    case FK_CG_B:
+    case FK_CG_RU:
+      if (kind == FK_CG_A) cgm_fin_a(st, sums[0], sums[1]);
+      else if (kind == FK_CG_B) cg_fin_b_rel(st, sums[0]);
+      else cg_fin_ru(st, sums[0]);
+      if (ctl && st->phase == PH_DONE) atomicAdd(&ctl->ndone, 1);
+      break;
   }
 }
 
@@ -333,6 +351,61 @@ __global__ __launch_bounds__(kBlock) void k_fast_switch(SolverState* st, int64_t
   }
 }
 
+// ---- mixed CG with reliable updates (grid.y = RHS) -------------------------
+// init: r_lo = b, p_lo = 0, x_lo = 0, x_hi = 0; partial |b|^2 (FP64)
+__global__ __launch_bounds__(kBlock) void k_rel_init(int64_t n, const Field* bs, float2* __restrict__ r,
+                                                     float2* __restrict__ p, float2* __restrict__ x,
+                                                     double2* __restrict__ x64, double* partials) {
+  const int rhs = blockIdx.y;
+  const double2* __restrict__ b = (const double2*)bs[rhs].data;
+  r += rhs * n;
+  p += rhs * n;
+  x += rhs * n;
+  x64 += rhs * n;
+  partials += (int64_t)rhs * gridDim.x;
+  double red[1] = {0.0};
+  LYM_GRID_STRIDE(i, n) {
+    const double2 v = b[i];
+    red[0] += v.x * v.x + v.y * v.y;
+    r[i] = make_float2((float)v.x, (float)v.y);
+    p[i] = make_float2(0.f, 0.f);
+    x[i] = make_float2(0.f, 0.f);
+    x64[i] = make_double2(0.0, 0.0);
+  }
+  block_partials<1>(red, partials);
+}
+
+// reliable update, accumulation (gate RU): x_hi += x_lo; x_lo = 0
+__global__ __launch_bounds__(kBlock) void k_ru_add(const SolverState* st, int64_t n, float2* __restrict__ x,
+                                                   double2* __restrict__ x64) {
+  const int rhs = blockIdx.y;
+  if (st[rhs].phase != PH_RU) return;
+  x += rhs * n;
+  x64 += rhs * n;
+  LYM_GRID_STRIDE(i, n) {
+    const float2 v = x[i];
+    double2 w = x64[i];
+    w.x += (double)v.x;
+    w.y += (double)v.y;
+    x64[i] = w;
+    x[i] = make_float2(0.f, 0.f);
+  }
+}
+
+// result: xs[rhs] = x_hi + x_lo (x_lo is zero after a final update)
+__global__ __launch_bounds__(kBlock) void k_rel_out(int64_t n, const float2* __restrict__ x,
+                                                    const double2* __restrict__ x64, const Field* xs) {
+  const int rhs = blockIdx.y;
+  double2* __restrict__ out = (double2*)xs[rhs].data;
+  x += rhs * n;
+  x64 += rhs * n;
+  LYM_GRID_STRIDE(i, n) {
+    const float2 v = x[i];
+    const double2 w = x64[i];
+    out[i] = make_double2(w.x + (double)v.x, w.y + (double)v.y);
+  }
+}
+
 // ---- MR (complex128) -------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_mr_init(int64_t n, const double2* __restrict__ b, double2* __restrict__ r,
                                                     const Field* xs, int first, int nshift) {
@@ -449,6 +522,9 @@ struct Parts {
   Field plive64, plive32;   // live base direction
   Field tb64, tb32;         // t = Q p
   Field ps64;               // nrhs x nshift FP64 shift directions
+  // mixed CG with reliable updates (one shift), every field nrhs x batched:
+  Field rl_r, rl_p, rl_t, rl_x;  // complex64 residual, direction, Q p, solution since the last update
+  Field rl_x64, rl_t64;          // complex128 accumulated solution, Q x
 };
 
 // prec: P64 = plain FP64 solves (and MR); otherwise the fast mixed solver
@@ -476,13 +552,23 @@ static size_t carve(const OpDesc& d, int prec, int nshift, int nrhs, char* base,
     P.t_lo = take_field(c, d, P32);
     P.x_lo[0] = take_field(c, d, P32);
     P.p[0] = take_field(c, d, P32);
-    P.ring64.data = c.take(field_bytes(d, P64) * (kFastBlock64 + 1) * nrhs);
-    P.ring32.data = c.take(field_bytes(d, P32) * (kFastBlock32 + 1) * nrhs);
-    P.plive64.data = c.take(field_bytes(d, P64) * nrhs);
-    P.plive32.data = c.take(field_bytes(d, P32) * nrhs);
-    P.tb64.data = c.take(field_bytes(d, P64) * nrhs);
-    P.tb32.data = c.take(field_bytes(d, P32) * nrhs);
-    P.ps64.data = c.take(field_bytes(d, P64) * nshift * nrhs);
+    if (nshift == 1) {
+      // one shift: CG with reliable updates
+      P.rl_r.data = c.take(field_bytes(d, P32) * nrhs);
+      P.rl_p.data = c.take(field_bytes(d, P32) * nrhs);
+      P.rl_t.data = c.take(field_bytes(d, P32) * nrhs);
+      P.rl_x.data = c.take(field_bytes(d, P32) * nrhs);
+      P.rl_x64.data = c.take(field_bytes(d, P64) * nrhs);
+      P.rl_t64.data = c.take(field_bytes(d, P64) * nrhs);
+    } else {
+      P.ring64.data = c.take(field_bytes(d, P64) * (kFastBlock64 + 1) * nrhs);
+      P.ring32.data = c.take(field_bytes(d, P32) * (kFastBlock32 + 1) * nrhs);
+      P.plive64.data = c.take(field_bytes(d, P64) * nrhs);
+      P.plive32.data = c.take(field_bytes(d, P32) * nrhs);
+      P.tb64.data = c.take(field_bytes(d, P64) * nrhs);
+      P.tb32.data = c.take(field_bytes(d, P32) * nrhs);
+      P.ps64.data = c.take(field_bytes(d, P64) * nshift * nrhs);
+    }
   }
   return c.off + 256;
 }
@@ -793,6 +879,126 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   return LYM_OK;
 }
 
+// One-shift CG on Q^2 + sigma over a batch of nrhs sources with complex64
+// Krylov vectors and FP64 reliable updates (Sleijpen & van der Vorst; the
+// scheme of Clark et al., arXiv:0911.3191): the low-precision recurrence runs
+// until its residual has dropped by 10x since the last update (or passes the
+// stopping test), then x_hi += x_lo and r = b - (Q^2 + sigma) x_hi are formed
+// in FP64 and replace the iterated residual, p is kept.  Every RHS advances
+// in the same launches; one that has stopped skips them.  xs[r] (FP64) get
+// the solutions.  Iteration (each line one launch, RHS batched):
+//   p = r + beta p ; t = Q p (|t|^2, |p|^2) ; alpha ;
+//   r -= alpha (Q t + sigma p), x_lo += alpha p (|r|^2) ; beta / update test ;
+//   [update: x_hi += x_lo ; t = Q x_hi ; r = b - (Q t + sigma x_hi) (|r|^2) ; test]
+constexpr int kRelChunk = 8;  // iterations per host chunk
+
+static int cg_reliable(const OpDesc& d, const Parts& P, const void* const* bs, void* const* xs, int nrhs, double sigma,
+                       double tol, int max_iter, hipStream_t st, std::vector<SolverState>& out_states) {
+  SolverState h;
+  const double tol_iter = 0.5 * tol;  // iterate below the target; the true residual decides
+  init_cgm_state(h, 1, &sigma, &tol_iter, max_iter);
+  h.phase = PH_KP;
+  h.delta2 = 0.01;
+  h.tol2_true = tol * tol;
+  std::vector<SolverState> hs(nrhs, h);
+  std::vector<Field> hb(nrhs), hx(nrhs);
+  for (int r = 0; r < nrhs; ++r) {
+    hb[r].data = const_cast<void*>(bs[r]);
+    hx[r].data = xs[r];
+  }
+  BatchCtl ctl{0, 0, nrhs, 0};
+  LYM_HIP(hipMemcpyAsync(P.st, hs.data(), sizeof(SolverState) * nrhs, hipMemcpyHostToDevice, st));
+  LYM_HIP(hipMemcpyAsync(P.ctl, &ctl, sizeof(ctl), hipMemcpyHostToDevice, st));
+  LYM_HIP(hipMemcpyAsync(P.bs_dev, hb.data(), sizeof(Field) * nrhs, hipMemcpyHostToDevice, st));
+  LYM_HIP(hipMemcpyAsync(P.xs_dev, hx.data(), sizeof(Field) * nrhs, hipMemcpyHostToDevice, st));
+  const int64_t n = field_elems(d);
+  const int gr = red_grid_flat(n);
+  const int gf = flat_grid(n);
+  hipLaunchKernelGGL(k_rel_init, dim3(gr, nrhs), dim3(kBlock), 0, st, n, P.bs_dev, (float2*)P.rl_r.data,
+                     (float2*)P.rl_p.data, (float2*)P.rl_x.data, (double2*)P.rl_x64.data, P.partials);
+  LYM_HIP(hipGetLastError());
+  int rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
+  if (rc) return rc;
+  const int gw32 = wilson_blocks(d, P32), gw64 = wilson_blocks(d, P64);
+  auto chunk = [&]() -> int {
+    for (int j = 0; j < kRelChunk; ++j) {
+      // p = r + beta p: the fast CG-M kernel with a one-slot ring
+      hipLaunchKernelGGL(k_fast_kp<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const float2*)P.rl_r.data,
+                         n, 1, (float2*)P.rl_p.data);
+      LYM_HIP(hipGetLastError());
+      EpiArgs a1;
+      a1.out = P.rl_t;
+      a1.st = P.st;
+      a1.rhs_stride = n;
+      int e = wilson_launch(EPI_CGM_K1, d, P32, true, P.rl_p, a1, redctl(P), st, nrhs);
+      if (!e) e = launch_fin(FK_CG_A, P.st, P.partials, gw32, 2, nullptr, st, nrhs, P.ctl);
+      if (e) return e;
+      EpiArgs a2;
+      a2.out = P.rl_x;
+      a2.aux0 = P.rl_p;
+      a2.aux1 = P.rl_r;
+      a2.st = P.st;
+      a2.rhs_stride = n;
+      e = wilson_launch(EPI_CG_K2X, d, P32, true, P.rl_t, a2, redctl(P), st, nrhs);
+      if (!e) e = launch_fin(FK_CG_B, P.st, P.partials, gw32, 1, nullptr, st, nrhs, P.ctl);
+      if (e) return e;
+      // reliable update: every kernel below returns at once for RHS not in PH_RU
+      hipLaunchKernelGGL(k_ru_add, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (float2*)P.rl_x.data,
+                         (double2*)P.rl_x64.data);
+      LYM_HIP(hipGetLastError());
+      EpiArgs a3;
+      a3.out = P.rl_t64;
+      a3.st = P.st;
+      a3.rhs_stride = n;
+      e = wilson_launch(EPI_RU1, d, P64, true, P.rl_x64, a3, RedCtl{nullptr, nullptr}, st, nrhs);
+      if (e) return e;
+      EpiArgs a4;
+      a4.out = P.rl_r;
+      a4.aux0 = P.rl_x64;
+      a4.bs = P.bs_dev;
+      a4.st = P.st;
+      a4.rhs_stride = n;
+      e = wilson_launch(EPI_RU2, d, P64, true, P.rl_t64, a4, redctl(P), st, nrhs);
+      if (!e) e = launch_fin(FK_CG_RU, P.st, P.partials, gw64, 1, nullptr, st, nrhs, P.ctl);
+      if (e) return e;
+    }
+    return LYM_OK;
+  };
+  rc = ensure_sync();
+  if (rc) return rc;
+  // an update takes an iteration slot without advancing iter: at most one per iteration
+  const int64_t max_chunks = 2 * ((int64_t)max_iter + kRelChunk) / kRelChunk + 8;
+  int64_t issued = 0, polled = 0;
+  auto issue = [&](int slot) -> int {
+    const int e = chunk();
+    if (e) return e;
+    LYM_HIP(hipMemcpyAsync(&g_sync.pinned[4 * slot], P.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    LYM_HIP(hipEventRecord(g_sync.ev[slot], st));
+    ++issued;
+    return LYM_OK;
+  };
+  if ((rc = issue(0))) return rc;
+  if ((rc = issue(1))) return rc;
+  while (true) {
+    const int slot = (int)(polled & 1);
+    LYM_HIP(hipEventSynchronize(g_sync.ev[slot]));
+    ++polled;
+    if (g_sync.pinned[4 * slot] >= nrhs) break;
+    if (issued > max_chunks) {
+      set_error("mixed CG did not reach its stop state (internal error)");
+      return LYM_ERR_HIP;
+    }
+    if ((rc = issue(slot))) return rc;
+  }
+  hipLaunchKernelGGL(k_rel_out, dim3(gf, nrhs), dim3(kBlock), 0, st, n, (const float2*)P.rl_x.data,
+                     (const double2*)P.rl_x64.data, P.xs_dev);
+  LYM_HIP(hipGetLastError());
+  out_states.resize(nrhs);
+  LYM_HIP(hipMemcpyAsync(out_states.data(), P.st, sizeof(SolverState) * nrhs, hipMemcpyDeviceToHost, st));
+  LYM_HIP(hipStreamSynchronize(st));
+  return LYM_OK;
+}
+
 static void fill_stats(lym_solve_stats* s, const SolverState& h, int nshift) {
   if (!s) return;
   s->status = h.status;
@@ -914,7 +1120,11 @@ int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const dou
     // mixed: FP64 Krylov vectors until every base residual is below 1e-4 of
     // its |b|, FP32 after; shift solutions and directions accumulate in FP64
     // through deferred block updates.  FP64 true residuals are checked below.
-    rc = cgm_fast(d, P, bs, xs, nrhs, nshift, shifts, tol_rel.data(), max_iter, 1e-4, P32, st, hs);
+    // One shift: complex64 CG with FP64 reliable updates instead.
+    if (nshift == 1)
+      rc = cg_reliable(d, P, bs, xs, nrhs, shifts[0], tol, max_iter, st, hs);
+    else
+      rc = cgm_fast(d, P, bs, xs, nrhs, nshift, shifts, tol_rel.data(), max_iter, 1e-4, P32, st, hs);
     if (rc) return rc;
   }
   bool breakdown = false;
@@ -933,7 +1143,7 @@ int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const dou
       lym_solve_stats* s = &stats[r];
       fill_stats(s, hs[r], nshift);
       s->status = status;
-      s->restarts = restarts;
+      s->restarts = restarts + hs[r].n_ru;  // refinement cycles + reliable updates
       for (int k = 0; k < nshift && k < 64; ++k) s->true_rel_resid[k] = rel[(size_t)r * nshift + k];
       s->seconds = secs;
     }

@@ -267,6 +267,122 @@ struct EpiMrK2 {
   }
 };
 
+// Mixed CG step 2 (low precision): Ap = Q t + s0 p; r <- r - alpha Ap;
+// x_lo <- x_lo + alpha p; reduce |r|^2.
+template <int P, int NCOMP>
+struct EpiCgK2X {
+  using R = typename Store<P>::R;
+  static constexpr int NRED = 1;
+  SpinorView<P> p, r, x;
+  SolverState* st;
+  int64_t stride;
+  EpiCgK2X(const EpiArgs& a, int64_t V)
+      : p(field_view<P>(a.aux0, V)), r(field_view<P>(a.aux1, V)), x(field_view<P>(a.out, V)), st(a.st),
+        stride(a.rhs_stride) {}
+  __device__ EpiCgK2X at(int rhs) const {
+    EpiCgK2X e = *this;
+    e.p = rhs_view(p, rhs, stride);
+    e.r = rhs_view(r, rhs, stride);
+    e.x = rhs_view(x, rhs, stride);
+    e.st = st + rhs;
+    return e;
+  }
+  __device__ bool active() const { return st->phase == PH_K2; }
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
+                        double* red) const {
+    cpx<R> c[NCOMP], res[NCOMP], pv[NCOMP], v[NCOMP];
+    in.template load<NCOMP>(s, c);
+    p.template load<NCOMP>(s, pv);
+    wilson_finish<NCOMP, R>(lat, c, hop, res);
+    const R s0 = (R)st->sigma0;
+    const R al = (R)st->alpha;
+    r.template load<NCOMP>(s, v);
+#pragma unroll
+    for (int k = 0; k < NCOMP; ++k) {
+      const R apr = res[k].re + s0 * pv[k].re, api = res[k].im + s0 * pv[k].im;
+      v[k] = mk<R>(v[k].re - al * apr, v[k].im - al * api);
+    }
+    r.template store<NCOMP>(s, v);
+    red[0] += norm2<NCOMP, R>(v);
+    x.template load<NCOMP>(s, v);
+#pragma unroll
+    for (int k = 0; k < NCOMP; ++k) v[k] = mk<R>(v[k].re + al * pv[k].re, v[k].im + al * pv[k].im);
+    x.template store<NCOMP>(s, v);
+  }
+};
+
+// Reliable update, first half (complex128, phase gated): t = Q x_hi.
+template <int P, int NCOMP>
+struct EpiRu1 {
+  using R = typename Store<P>::R;
+  static constexpr int NRED = 0;
+  SpinorView<P> out;
+  SolverState* st;
+  int64_t stride;
+  EpiRu1(const EpiArgs& a, int64_t V) : out(field_view<P>(a.out, V)), st(a.st), stride(a.rhs_stride) {}
+  __device__ EpiRu1 at(int rhs) const {
+    EpiRu1 e = *this;
+    e.out = rhs_view(out, rhs, stride);
+    e.st = st + rhs;
+    return e;
+  }
+  __device__ bool active() const { return st->phase == PH_RU; }
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
+                        double*) const {
+    cpx<R> c[NCOMP], res[NCOMP];
+    in.template load<NCOMP>(s, c);
+    wilson_finish<NCOMP, R>(lat, c, hop, res);
+    out.template store<NCOMP>(s, res);
+  }
+};
+
+// Reliable update, second half (complex128): r = b - (Q t + s0 x_hi), the
+// true residual; reduce |r|^2 in FP64 and store r in complex64 as the new
+// low-precision residual.  b of RHS i is bs[i] (caller's arrays).
+template <int P, int NCOMP>
+struct EpiRu2 {
+  using R = typename Store<P>::R;
+  static constexpr int NRED = 1;
+  SpinorView<P> x, b;
+  SpinorView<P32> rlo;
+  const Field* bs;
+  SolverState* st;
+  int64_t stride;
+  EpiRu2(const EpiArgs& a, int64_t V)
+      : x(field_view<P>(a.aux0, V)), b(field_view<P>(Field{}, V)), rlo(field_view<P32>(a.out, V)), bs(a.bs),
+        st(a.st), stride(a.rhs_stride) {}
+  __device__ EpiRu2 at(int rhs) const {
+    EpiRu2 e = *this;
+    e.x = rhs_view(x, rhs, stride);
+    e.rlo = rhs_view(rlo, rhs, stride);
+    e.b.data = (typename Store<P>::elem*)bs[rhs].data;
+    e.st = st + rhs;
+    return e;
+  }
+  __device__ bool active() const { return st->phase == PH_RU; }
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
+                        double* red) const {
+    cpx<R> c[NCOMP], res[NCOMP], xv[NCOMP];
+    in.template load<NCOMP>(s, c);
+    x.template load<NCOMP>(s, xv);
+    wilson_finish<NCOMP, R>(lat, c, hop, res);
+    const R s0 = (R)st->sigma0;
+    b.template load<NCOMP>(s, c);
+    cpx<float> lo[NCOMP];
+#pragma unroll
+    for (int k = 0; k < NCOMP; ++k) {
+      const R rr = c[k].re - (res[k].re + s0 * xv[k].re);
+      const R ri = c[k].im - (res[k].im + s0 * xv[k].im);
+      red[0] += (double)rr * rr + (double)ri * ri;
+      lo[k] = mk<float>((float)rr, (float)ri);
+    }
+    rlo.template store<NCOMP>(s, lo);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // The kernel.  gs = site blocks; the grid has gs * nrhs blocks.
 // ---------------------------------------------------------------------------
@@ -550,6 +666,15 @@ int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, 
     case EPI_CGM_K1: return launch_prec<EpiCgmK1>(prec, d, proj, in, a, rc, stream, nrhs);
     case EPI_CGM_K2: return launch_prec<EpiCgmK2>(prec, d, proj, in, a, rc, stream, nrhs);
     case EPI_CGM_K2F: return launch_prec<EpiCgmK2F>(prec, d, proj, in, a, rc, stream, nrhs);
+    case EPI_CG_K2X: return launch_prec<EpiCgK2X>(prec, d, proj, in, a, rc, stream, nrhs);
+    case EPI_RU1:
+    case EPI_RU2:
+      if (prec != P64) {
+        set_error("the reliable update runs in complex128");
+        return LYM_ERR_UNSUPPORTED;
+      }
+      if (kind == EPI_RU1) return launch_geom<P64, EpiRu1>(d, proj, in, a, rc, stream, nrhs);
+      return launch_geom<P64, EpiRu2>(d, proj, in, a, rc, stream, nrhs);
     case EPI_MR_K1:
     case EPI_MR_K2:
       if (prec != P64 || nrhs != 1) {
