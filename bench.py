@@ -34,9 +34,11 @@ WORKLOADS = {
 BATCHED = {
     # 12 point sources (one per spin-colour at the origin) + 16 Z2-noise sources,
     # CG on D^dag D (shift 0) to true relative residual 1e-10
-    "c3b": dict(geometry=(24, 24, 24, 48), nc=3, kappa=0.1335, eps=0.25, npoint=12, nz2=16,
+    # (precision: half = FP16 Krylov storage with per-site norms, FP32
+    # arithmetic, FP64 reliable updates)
+    "c3b": dict(geometry=(24, 24, 24, 48), nc=3, kappa=0.1335, eps=0.25, npoint=12, nz2=16, precision="mixed",
                 name="wilson_ddag_d_24c3x48_su3_batched28"),
-    "c2b": dict(geometry=(16, 16, 16, 32), nc=3, kappa=0.13, eps=0.3, npoint=12, nz2=16,
+    "c2b": dict(geometry=(16, 16, 16, 32), nc=3, kappa=0.13, eps=0.3, npoint=12, nz2=16, precision="mixed",
                 name="wilson_ddag_d_16c3x32_su3_batched28"),
 }
 
@@ -129,7 +131,8 @@ def run_batched(args):
     op = dirac.DiracWilsonOperator(lattice.Configuration(U, geometry, nc, 4), W["kappa"], True,
                                    representation="fundamental")
     A = dirac.SquareOperator(op)
-    solver = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="mixed",
+    mode = args.precision or W["precision"]
+    solver = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode=mode,
                                             verbose=False)
     for _ in range(args.warmup):
         solver.solve_batch(A, srcs, [0.0])
@@ -164,9 +167,12 @@ def run_batched(args):
         "metric": "batched multi-RHS CG solves/s at 1e-10 true resid",
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": total / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded near-identity SU(3) links; point + Z2 sources)",
+        "vs_baseline": None,
+        "dtype": {"half": "f16 storage + f32 arithmetic + f64 reliable updates", "mixed": "f32+f64",
+                  "double": "f64"}[mode],
+        "data": "synthetic (seeded near-identity SU(3) links; point + Z2 sources)",
         "config": {"workload": W["name"], "geometry": list(geometry), "kappa": W["kappa"], "rhs": len(srcs),
-                   "tolerance": 1e-10},
+                   "precision": mode, "tolerance": 1e-10},
         "iterations": solver.last_stats[0]["iterations"],
         "plain_fp64_single_rhs_solves_per_s": plain,
         "speedup_vs_plain_fp64": (value / plain) if plain else None,
@@ -190,7 +196,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(SERVING))
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "double"])
+    ap.add_argument("--precision", default=None, choices=["mixed", "double", "half"],
+                    help="Krylov precision (default: mixed; batched workloads: their own)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--dslash-reps", type=int, default=50)
@@ -204,6 +211,8 @@ def main():
         sys.exit(2)
     if args.workload in BATCHED:
         return run_batched(args)
+    if args.precision is None:
+        args.precision = "mixed"
     if args.workload in SERVING:
         return run_serving(args)
 

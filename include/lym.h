@@ -153,8 +153,17 @@ int lym_mr_multishift(const lym_wilson* op, int nshift, const double* shifts, in
  * iterated relative residual <= tol; then recomputes the FP64 true relative
  * residual of every shift into stats->true_rel_resid.
  *   prec = LYM_PREC_F64: FP64 throughout.
- *   prec = LYM_PREC_F32: FP32 CG-M to tol_inner, then FP64 defect
- *          correction of every shift (FP32 inner CG) to tol.
+ *   prec = LYM_PREC_F32, several shifts: FP64 Krylov vectors until the base
+ *          residual is below 1e-4 |b|, complex64 after; shift solutions and
+ *          directions accumulate in FP64 (deferred block updates); FP64
+ *          defect correction of any shift whose true residual misses tol.
+ *   prec = LYM_PREC_F32, one shift: complex64 CG with FP64 reliable updates
+ *          (true residual recomputed in FP64 whenever the iterated one has
+ *          dropped 10x, and at the end).
+ *   prec = LYM_PREC_F16, one shift, fundamental representation: the same
+ *          with 16-bit storage of the Krylov vectors (int16 fixed point
+ *          scaled by a per-site float norm, 4 bytes per complex number);
+ *          arithmetic in FP32.
  * b and xs are complex128.  Synchronous.  New entry point (no counterpart in
  * the reference, whose multishift solver is the MR above); it serves
  * multishift_solver.solve / RationalApproximation.apply (dirac.py:52-59). */

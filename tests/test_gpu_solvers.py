@@ -109,6 +109,29 @@ def test_mixed_cg_reliable_updates(shift):
         assert torch.equal(row[0], single[0])
 
 
+@pytest.mark.parametrize("geometry,nc", [((8, 4, 4, 16), 3), ((4, 4, 4, 8), 2), ((8, 8), 3)])
+def test_half_storage_cg(geometry, nc):
+    """One shift with half-precision Krylov storage (per-site norms) and FP64
+    reliable updates: every source reaches 1e-10 (oracle-recomputed) and
+    matches the FP64 oracle CG; batched and single solves give the same bits."""
+    from tfleonardym_amd import dirac, dirac_solver
+    print(f"case half CG geometry={geometry} nc={nc}", flush=True)
+    op, oop = setup(geometry, nc, 0.13)
+    ns = 2 if len(geometry) == 2 else 4
+    srcs = [fields.gaussian_spinor(geometry, ns, nc, seed=70), fields.z2_noise(geometry, ns, nc, seed=71)]
+    s = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="half",
+                                       verbose=False)
+    sols = s.solve_batch(dirac.SquareOperator(op), srcs, [0.01])
+    assert s.last_status == "converged", s.last_stats
+    for b, row in zip(srcs, sols):
+        xn = row[0].cpu().numpy()
+        assert osv.true_residual(oop.apply_square, b, xn, 0.01) <= 1e-10
+        xr, _ = osv.cg(oop.apply_square, b, 0.01, tol=1e-12)
+        assert rel(xn, xr) < 1e-8
+        single = s.solve(dirac.SquareOperator(op), b, [0.01])
+        assert torch.equal(row[0], single[0])
+
+
 def test_mixed_cg_max_steps():
     from tfleonardym_amd import dirac, dirac_solver
     op, _ = setup((4, 4, 4, 4), 3, 0.13)

@@ -158,14 +158,19 @@ __device__ inline void cg_fin_b_rel(SolverState* st, double rrn) {
     return;
   }
   st->beta_old = rrn / st->rr;
+  st->kp_a_re = 1.0;
+  st->kp_a_im = 0.0;
   st->rr = rrn;
   st->phase = PH_KP;
 }
 
 // After the FP64 true residual of a reliable update: converged when it meets
 // the target, stopped at the iteration budget, else the low-precision
-// recurrence restarts from it with p = r_true + (rr_true / rr_k) p.
-__device__ inline void cg_fin_ru(SolverState* st, double rr_true) {
+// recurrence restarts from it with p = r_true + beta (p - c r_true),
+// beta = rr_true / rr_k, c = <r_true, p> / rr_true: the kept direction is
+// made orthogonal to the new residual again (as in Clark et al.,
+// arXiv:0911.3191), which low-precision storage of p otherwise erodes.
+__device__ inline void cg_fin_ru(SolverState* st, double rr_true, double rp_re, double rp_im) {
   st->n_ru += 1;
   if (!isfinite(rr_true)) {
     st->status = ST_BREAKDOWN;
@@ -183,7 +188,10 @@ __device__ inline void cg_fin_ru(SolverState* st, double rr_true) {
     st->phase = PH_DONE;
     return;
   }
-  st->beta_old = rr_true / st->rr;
+  const double beta = rr_true / st->rr;
+  st->beta_old = beta;
+  st->kp_a_re = 1.0 - beta * rp_re / rr_true;
+  st->kp_a_im = -beta * rp_im / rr_true;
   st->rr = rr_true;
   st->rr_max = rr_true;
   st->phase = PH_KP;

@@ -77,12 +77,14 @@ enum EpiKind : int {
   EPI_CG_K2X = 7,    // mixed CG: aux1 -= alpha (Op in + sigma0 aux0); out += alpha aux0; reduce |aux1|^2
   EPI_RU1 = 8,       // reliable update (complex128, gate RU): out = Op in
   EPI_RU2 = 9,       // reliable update: r = bs - (Op in + sigma0 aux0) -> out (complex64); reduce |r|^2
+  EPI_RU2H = 10,     // the same with out in half storage (per-site norms)
 };
 int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, const EpiArgs& a, RedCtl rc,
                   hipStream_t stream, int nrhs = 1);
 // number of site blocks (= partials per RHS) of a Wilson launch at `prec`
 inline bool wilson_tiled(const OpDesc& d, int prec) {
-  return d.use_tile && (prec == P64 || prec == P32) && d.tile[prec == P64 ? 0 : 1].on;
+  // half storage decodes into the complex64 tile (same LDS footprint)
+  return d.use_tile && (prec == P64 || prec == P32 || prec == P16) && d.tile[prec == P64 ? 0 : 1].on;
 }
 inline int wilson_blocks(const OpDesc& d, int prec) {
   return wilson_tiled(d, prec) ? d.tile[prec == P64 ? 0 : 1].blocks : d.wblocks;

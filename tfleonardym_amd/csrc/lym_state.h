@@ -70,6 +70,10 @@ struct SolverState {
   double rr_max;
   double delta2;
   double tol2_true;  // target on the true rr (absolute after the init scaling)
+  // next direction p = kp_a r + beta_old p; kp_a = 1 except right after a
+  // reliable update, where it also removes the component of p along the new
+  // residual (kp_a = 1 - beta <r,p>/<r,r>)
+  double kp_a_re, kp_a_im;
   int ru_final;  // the pending update follows the iterated stopping test
   int n_ru;      // reliable updates done
 };

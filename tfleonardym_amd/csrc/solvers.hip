@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
     case FK_CG_RU:
       if (kind == FK_CG_A) cgm_fin_a(st, sums[0], sums[1]);
       else if (kind == FK_CG_B) cg_fin_b_rel(st, sums[0]);
-      else cg_fin_ru(st, sums[0]);
+      else cg_fin_ru(st, sums[0], sums[1], sums[2]);
       if (ctl && st->phase == PH_DONE) atomicAdd(&ctl->ndone, 1);
       break;
   }
@@ -271,12 +271,26 @@ __global__ __launch_bounds__(kBlock) void k_fast_kp(const SolverState* st, int64
   p += rhs * n;
   const E* r = ring + (int64_t)(st->iter % nslot) * n;
   const R b = (R)st->beta_old;
-  LYM_GRID_STRIDE(i, n) {
-    const E rv = r[i];
-    E q = p[i];
-    q.x = rv.x + b * q.x;
-    q.y = rv.y + b * q.y;
-    p[i] = q;
+  const R ar = (R)st->kp_a_re, ai = (R)st->kp_a_im;
+  if (ar == R(1) && ai == R(0)) {
+    LYM_GRID_STRIDE(i, n) {
+      const E rv = r[i];
+      E q = p[i];
+      q.x = rv.x + b * q.x;
+      q.y = rv.y + b * q.y;
+      p[i] = q;
+    }
+  } else {
+    // after a reliable update: p = kp_a r + beta p
+    LYM_GRID_STRIDE(i, n) {
+      const E rv = r[i];
+      E q = p[i];
+      const R qx = ar * rv.x - ai * rv.y + b * q.x;
+      const R qy = ar * rv.y + ai * rv.x + b * q.y;
+      q.x = qx;
+      q.y = qy;
+      p[i] = q;
+    }
   }
 }
 
@@ -403,6 +417,109 @@ __global__ __launch_bounds__(kBlock) void k_rel_out(int64_t n, const float2* __r
     const float2 v = x[i];
     const double2 w = x64[i];
     out[i] = make_double2(w.x + (double)v.x, w.y + (double)v.y);
+  }
+}
+
+// ---- the same steps with half storage (per-site norms): a thread owns whole
+// sites, since storing a site rescales all its components.  Fields of RHS i
+// start at data + i * NCOMP * V and norm + i * V.
+#define LYM_SITE_STRIDE(s, V) \
+  for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < (V); s += (int64_t)gridDim.x * kBlock)
+
+template <int NCOMP>
+__device__ __forceinline__ SpinorView<P16> half_view(const Field& f, int64_t V, int rhs) {
+  return SpinorView<P16>{(Store<P16>::elem*)f.data + (int64_t)rhs * NCOMP * V, f.norm + (int64_t)rhs * V, V};
+}
+
+template <int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_rel_init16(int64_t V, const Field* bs, Field r, Field p, Field x,
+                                                       double2* __restrict__ x64, double* partials) {
+  const int rhs = blockIdx.y;
+  const SpinorView<P64> b{(double2*)bs[rhs].data, nullptr, V};
+  const SpinorView<P16> rv = half_view<NCOMP>(r, V, rhs), pv = half_view<NCOMP>(p, V, rhs),
+                        xv = half_view<NCOMP>(x, V, rhs);
+  const SpinorView<P64> xh{x64 + (int64_t)rhs * NCOMP * V, nullptr, V};
+  partials += (int64_t)rhs * gridDim.x;
+  double red[1] = {0.0};
+  LYM_SITE_STRIDE(s, V) {
+    cpx<double> v[NCOMP];
+    b.template load<NCOMP>(s, v);
+    cpx<float> lo[NCOMP], z[NCOMP];
+    cpx<double> z64[NCOMP];
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) {
+      red[0] += v[c].re * v[c].re + v[c].im * v[c].im;
+      lo[c] = mk<float>((float)v[c].re, (float)v[c].im);
+      z[c] = czero<float>();
+      z64[c] = czero<double>();
+    }
+    rv.template store<NCOMP>(s, lo);
+    pv.template store<NCOMP>(s, z);
+    xv.template store<NCOMP>(s, z);
+    xh.template store<NCOMP>(s, z64);
+  }
+  block_partials<1>(red, partials);
+}
+
+// p = r + beta p (gate KP)
+template <int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_kp16(const SolverState* st, int64_t V, Field r, Field p) {
+  const int rhs = blockIdx.y;
+  st += rhs;
+  if (st->phase != PH_KP) return;
+  const SpinorView<P16> rv = half_view<NCOMP>(r, V, rhs), pv = half_view<NCOMP>(p, V, rhs);
+  const float b = (float)st->beta_old;
+  const float ar = (float)st->kp_a_re, ai = (float)st->kp_a_im;  // p = kp_a r + beta p
+  LYM_SITE_STRIDE(s, V) {
+    cpx<float> a[NCOMP], q[NCOMP];
+    rv.template load<NCOMP>(s, a);
+    pv.template load<NCOMP>(s, q);
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c)
+      q[c] = mk<float>(ar * a[c].re - ai * a[c].im + b * q[c].re, ar * a[c].im + ai * a[c].re + b * q[c].im);
+    pv.template store<NCOMP>(s, q);
+  }
+}
+
+// reliable update, accumulation (gate RU): x_hi += x_lo; x_lo = 0
+template <int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_ru_add16(const SolverState* st, int64_t V, Field x,
+                                                     double2* __restrict__ x64) {
+  const int rhs = blockIdx.y;
+  if (st[rhs].phase != PH_RU) return;
+  const SpinorView<P16> xv = half_view<NCOMP>(x, V, rhs);
+  const SpinorView<P64> xh{x64 + (int64_t)rhs * NCOMP * V, nullptr, V};
+  LYM_SITE_STRIDE(s, V) {
+    cpx<float> a[NCOMP], z[NCOMP];
+    cpx<double> w[NCOMP];
+    xv.template load<NCOMP>(s, a);
+    xh.template load<NCOMP>(s, w);
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) {
+      w[c] = mk<double>(w[c].re + (double)a[c].re, w[c].im + (double)a[c].im);
+      z[c] = czero<float>();
+    }
+    xh.template store<NCOMP>(s, w);
+    xv.template store<NCOMP>(s, z);
+  }
+}
+
+// result: xs[rhs] = x_hi + x_lo
+template <int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_rel_out16(int64_t V, Field x, const double2* __restrict__ x64,
+                                                      const Field* xs) {
+  const int rhs = blockIdx.y;
+  const SpinorView<P16> xv = half_view<NCOMP>(x, V, rhs);
+  const SpinorView<P64> xh{const_cast<double2*>(x64) + (int64_t)rhs * NCOMP * V, nullptr, V};
+  const SpinorView<P64> out{(double2*)xs[rhs].data, nullptr, V};
+  LYM_SITE_STRIDE(s, V) {
+    cpx<float> a[NCOMP];
+    cpx<double> w[NCOMP];
+    xv.template load<NCOMP>(s, a);
+    xh.template load<NCOMP>(s, w);
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) w[c] = mk<double>(w[c].re + (double)a[c].re, w[c].im + (double)a[c].im);
+    out.template store<NCOMP>(s, w);
   }
 }
 
@@ -553,13 +670,21 @@ static size_t carve(const OpDesc& d, int prec, int nshift, int nrhs, char* base,
     P.x_lo[0] = take_field(c, d, P32);
     P.p[0] = take_field(c, d, P32);
     if (nshift == 1) {
-      // one shift: CG with reliable updates
-      P.rl_r.data = c.take(field_bytes(d, P32) * nrhs);
-      P.rl_p.data = c.take(field_bytes(d, P32) * nrhs);
-      P.rl_t.data = c.take(field_bytes(d, P32) * nrhs);
-      P.rl_x.data = c.take(field_bytes(d, P32) * nrhs);
-      P.rl_x64.data = c.take(field_bytes(d, P64) * nrhs);
-      P.rl_t64.data = c.take(field_bytes(d, P64) * nrhs);
+      // one shift: CG with reliable updates, Krylov vectors in complex64 or
+      // half storage (per-site norms, nrhs x V floats)
+      const int pl = prec == P16 ? P16 : P32;
+      auto take_batch = [&](int q) {
+        Field f;
+        f.data = c.take(field_bytes(d, q) * nrhs);
+        if (q == P16) f.norm = (float*)c.take((size_t)d.lat.V * sizeof(float) * nrhs);
+        return f;
+      };
+      P.rl_r = take_batch(pl);
+      P.rl_p = take_batch(pl);
+      P.rl_t = take_batch(pl);
+      P.rl_x = take_batch(pl);
+      P.rl_x64 = take_batch(P64);
+      P.rl_t64 = take_batch(P64);
     } else {
       P.ring64.data = c.take(field_bytes(d, P64) * (kFastBlock64 + 1) * nrhs);
       P.ring32.data = c.take(field_bytes(d, P32) * (kFastBlock32 + 1) * nrhs);
@@ -677,6 +802,7 @@ static void init_cgm_state(SolverState& h, int nshift, const double* shifts, con
   h.base = base;
   h.sigma0 = shifts[base];
   h.alpha_old = 1.0;
+  h.kp_a_re = 1.0;
   h.beta_old = 0.0;
   for (int k = 0; k < nshift; ++k) {
     h.shift[k] = shifts[k];
@@ -892,8 +1018,24 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
 //   [update: x_hi += x_lo ; t = Q x_hi ; r = b - (Q t + sigma x_hi) (|r|^2) ; test]
 constexpr int kRelChunk = 8;  // iterations per host chunk
 
-static int cg_reliable(const OpDesc& d, const Parts& P, const void* const* bs, void* const* xs, int nrhs, double sigma,
-                       double tol, int max_iter, hipStream_t st, std::vector<SolverState>& out_states) {
+static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* const* bs, void* const* xs, int nrhs,
+                       double sigma, double tol, int max_iter, hipStream_t st, std::vector<SolverState>& out_states) {
+  // half storage: site kernels instantiated per component count
+  auto with_ncomp = [&](auto f) -> int {
+    switch (d.ncomp) {
+      case 4: f(IC<4>{}); break;
+      case 6: f(IC<6>{}); break;
+      case 8: f(IC<8>{}); break;
+      case 12: f(IC<12>{}); break;
+      default:
+        set_error("half-precision storage: fundamental representation only");
+        return LYM_ERR_UNSUPPORTED;
+    }
+    LYM_HIP(hipGetLastError());
+    return LYM_OK;
+  };
+  const int64_t V = d.lat.V;
+  const int gs = flat_grid(V);
   SolverState h;
   const double tol_iter = 0.5 * tol;  // iterate below the target; the true residual decides
   init_cgm_state(h, 1, &sigma, &tol_iter, max_iter);
@@ -914,24 +1056,42 @@ static int cg_reliable(const OpDesc& d, const Parts& P, const void* const* bs, v
   const int64_t n = field_elems(d);
   const int gr = red_grid_flat(n);
   const int gf = flat_grid(n);
-  hipLaunchKernelGGL(k_rel_init, dim3(gr, nrhs), dim3(kBlock), 0, st, n, P.bs_dev, (float2*)P.rl_r.data,
-                     (float2*)P.rl_p.data, (float2*)P.rl_x.data, (double2*)P.rl_x64.data, P.partials);
-  LYM_HIP(hipGetLastError());
-  int rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
+  int rc = LYM_OK;
+  if (pl == P16) {
+    rc = with_ncomp([&](auto nc) {
+      hipLaunchKernelGGL(k_rel_init16<decltype(nc)::value>, dim3(gr, nrhs), dim3(kBlock), 0, st, V, P.bs_dev,
+                         P.rl_r, P.rl_p, P.rl_x, (double2*)P.rl_x64.data, P.partials);
+    });
+  } else {
+    hipLaunchKernelGGL(k_rel_init, dim3(gr, nrhs), dim3(kBlock), 0, st, n, P.bs_dev, (float2*)P.rl_r.data,
+                       (float2*)P.rl_p.data, (float2*)P.rl_x.data, (double2*)P.rl_x64.data, P.partials);
+    LYM_HIP(hipGetLastError());
+  }
   if (rc) return rc;
-  const int gw32 = wilson_blocks(d, P32), gw64 = wilson_blocks(d, P64);
+  rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
+  if (rc) return rc;
+  const int gwl = wilson_blocks(d, pl), gw64 = wilson_blocks(d, P64);
   auto chunk = [&]() -> int {
     for (int j = 0; j < kRelChunk; ++j) {
-      // p = r + beta p: the fast CG-M kernel with a one-slot ring
-      hipLaunchKernelGGL(k_fast_kp<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const float2*)P.rl_r.data,
-                         n, 1, (float2*)P.rl_p.data);
-      LYM_HIP(hipGetLastError());
+      // p = r + beta p (complex64: the fast CG-M kernel with a one-slot ring)
+      int e = LYM_OK;
+      if (pl == P16) {
+        e = with_ncomp([&](auto nc) {
+          hipLaunchKernelGGL(k_kp16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, P.st, V, P.rl_r,
+                             P.rl_p);
+        });
+        if (e) return e;
+      } else {
+        hipLaunchKernelGGL(k_fast_kp<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
+                           (const float2*)P.rl_r.data, n, 1, (float2*)P.rl_p.data);
+        LYM_HIP(hipGetLastError());
+      }
       EpiArgs a1;
       a1.out = P.rl_t;
       a1.st = P.st;
       a1.rhs_stride = n;
-      int e = wilson_launch(EPI_CGM_K1, d, P32, true, P.rl_p, a1, redctl(P), st, nrhs);
-      if (!e) e = launch_fin(FK_CG_A, P.st, P.partials, gw32, 2, nullptr, st, nrhs, P.ctl);
+      e = wilson_launch(EPI_CGM_K1, d, pl, true, P.rl_p, a1, redctl(P), st, nrhs);
+      if (!e) e = launch_fin(FK_CG_A, P.st, P.partials, gwl, 2, nullptr, st, nrhs, P.ctl);
       if (e) return e;
       EpiArgs a2;
       a2.out = P.rl_x;
@@ -939,13 +1099,21 @@ static int cg_reliable(const OpDesc& d, const Parts& P, const void* const* bs, v
       a2.aux1 = P.rl_r;
       a2.st = P.st;
       a2.rhs_stride = n;
-      e = wilson_launch(EPI_CG_K2X, d, P32, true, P.rl_t, a2, redctl(P), st, nrhs);
-      if (!e) e = launch_fin(FK_CG_B, P.st, P.partials, gw32, 1, nullptr, st, nrhs, P.ctl);
+      e = wilson_launch(EPI_CG_K2X, d, pl, true, P.rl_t, a2, redctl(P), st, nrhs);
+      if (!e) e = launch_fin(FK_CG_B, P.st, P.partials, gwl, 1, nullptr, st, nrhs, P.ctl);
       if (e) return e;
       // reliable update: every kernel below returns at once for RHS not in PH_RU
-      hipLaunchKernelGGL(k_ru_add, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (float2*)P.rl_x.data,
-                         (double2*)P.rl_x64.data);
-      LYM_HIP(hipGetLastError());
+      if (pl == P16) {
+        e = with_ncomp([&](auto nc) {
+          hipLaunchKernelGGL(k_ru_add16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, P.st, V, P.rl_x,
+                             (double2*)P.rl_x64.data);
+        });
+        if (e) return e;
+      } else {
+        hipLaunchKernelGGL(k_ru_add, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (float2*)P.rl_x.data,
+                           (double2*)P.rl_x64.data);
+        LYM_HIP(hipGetLastError());
+      }
       EpiArgs a3;
       a3.out = P.rl_t64;
       a3.st = P.st;
@@ -955,11 +1123,12 @@ static int cg_reliable(const OpDesc& d, const Parts& P, const void* const* bs, v
       EpiArgs a4;
       a4.out = P.rl_r;
       a4.aux0 = P.rl_x64;
+      a4.aux1 = P.rl_p;
       a4.bs = P.bs_dev;
       a4.st = P.st;
       a4.rhs_stride = n;
-      e = wilson_launch(EPI_RU2, d, P64, true, P.rl_t64, a4, redctl(P), st, nrhs);
-      if (!e) e = launch_fin(FK_CG_RU, P.st, P.partials, gw64, 1, nullptr, st, nrhs, P.ctl);
+      e = wilson_launch(pl == P16 ? EPI_RU2H : EPI_RU2, d, P64, true, P.rl_t64, a4, redctl(P), st, nrhs);
+      if (!e) e = launch_fin(FK_CG_RU, P.st, P.partials, gw64, 3, nullptr, st, nrhs, P.ctl);
       if (e) return e;
     }
     return LYM_OK;
@@ -990,9 +1159,17 @@ static int cg_reliable(const OpDesc& d, const Parts& P, const void* const* bs, v
     }
     if ((rc = issue(slot))) return rc;
   }
-  hipLaunchKernelGGL(k_rel_out, dim3(gf, nrhs), dim3(kBlock), 0, st, n, (const float2*)P.rl_x.data,
-                     (const double2*)P.rl_x64.data, P.xs_dev);
-  LYM_HIP(hipGetLastError());
+  if (pl == P16) {
+    rc = with_ncomp([&](auto nc) {
+      hipLaunchKernelGGL(k_rel_out16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, V, P.rl_x,
+                         (const double2*)P.rl_x64.data, P.xs_dev);
+    });
+    if (rc) return rc;
+  } else {
+    hipLaunchKernelGGL(k_rel_out, dim3(gf, nrhs), dim3(kBlock), 0, st, n, (const float2*)P.rl_x.data,
+                       (const double2*)P.rl_x64.data, P.xs_dev);
+    LYM_HIP(hipGetLastError());
+  }
   out_states.resize(nrhs);
   LYM_HIP(hipMemcpyAsync(out_states.data(), P.st, sizeof(SolverState) * nrhs, hipMemcpyDeviceToHost, st));
   LYM_HIP(hipStreamSynchronize(st));
@@ -1089,8 +1266,8 @@ int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const dou
       set_error("CG-M needs non-negative shifts");
       return LYM_ERR_ARG;
     }
-  if (prec == P16) {
-    set_error("half-precision CG-M is not built yet");
+  if (prec == P16 && (nshift != 1 || d.adj)) {
+    set_error("half-precision storage is built for one-shift CG in the fundamental representation");
     return LYM_ERR_UNSUPPORTED;
   }
   const int nbatch = prec == P64 ? 1 : nrhs;  // plain FP64 runs one RHS at a time
@@ -1122,7 +1299,7 @@ int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const dou
     // through deferred block updates.  FP64 true residuals are checked below.
     // One shift: complex64 CG with FP64 reliable updates instead.
     if (nshift == 1)
-      rc = cg_reliable(d, P, bs, xs, nrhs, shifts[0], tol, max_iter, st, hs);
+      rc = cg_reliable(d, P, prec == P16 ? P16 : P32, bs, xs, nrhs, shifts[0], tol, max_iter, st, hs);
     else
       rc = cgm_fast(d, P, bs, xs, nrhs, nshift, shifts, tol_rel.data(), max_iter, 1e-4, P32, st, hs);
     if (rc) return rc;

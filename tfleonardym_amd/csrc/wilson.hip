@@ -339,24 +339,26 @@ struct EpiRu1 {
 };
 
 // Reliable update, second half (complex128): r = b - (Q t + s0 x_hi), the
-// true residual; reduce |r|^2 in FP64 and store r in complex64 as the new
-// low-precision residual.  b of RHS i is bs[i] (caller's arrays).
-template <int P, int NCOMP>
-struct EpiRu2 {
+// true residual; reduce |r|^2 in FP64 and store r at the low precision PL
+// (complex64 or half + per-site norm) as the new iterated residual.  b of
+// RHS i is bs[i] (caller's arrays).
+template <int P, int NCOMP, int PL>
+struct EpiRu2Base {
   using R = typename Store<P>::R;
-  static constexpr int NRED = 1;
+  static constexpr int NRED = 3;  // |r|^2, <r, p> (re, im)
   SpinorView<P> x, b;
-  SpinorView<P32> rlo;
+  SpinorView<PL> rlo, p;
   const Field* bs;
   SolverState* st;
   int64_t stride;
-  EpiRu2(const EpiArgs& a, int64_t V)
-      : x(field_view<P>(a.aux0, V)), b(field_view<P>(Field{}, V)), rlo(field_view<P32>(a.out, V)), bs(a.bs),
-        st(a.st), stride(a.rhs_stride) {}
-  __device__ EpiRu2 at(int rhs) const {
-    EpiRu2 e = *this;
+  EpiRu2Base(const EpiArgs& a, int64_t V)
+      : x(field_view<P>(a.aux0, V)), b(field_view<P>(Field{}, V)), rlo(field_view<PL>(a.out, V)),
+        p(field_view<PL>(a.aux1, V)), bs(a.bs), st(a.st), stride(a.rhs_stride) {}
+  __device__ EpiRu2Base at(int rhs) const {
+    EpiRu2Base e = *this;
     e.x = rhs_view(x, rhs, stride);
     e.rlo = rhs_view(rlo, rhs, stride);
+    e.p = rhs_view(p, rhs, stride);
     e.b.data = (typename Store<P>::elem*)bs[rhs].data;
     e.st = st + rhs;
     return e;
@@ -371,17 +373,24 @@ struct EpiRu2 {
     wilson_finish<NCOMP, R>(lat, c, hop, res);
     const R s0 = (R)st->sigma0;
     b.template load<NCOMP>(s, c);
-    cpx<float> lo[NCOMP];
+    cpx<float> lo[NCOMP], pv[NCOMP];
+    p.template load<NCOMP>(s, pv);
 #pragma unroll
     for (int k = 0; k < NCOMP; ++k) {
       const R rr = c[k].re - (res[k].re + s0 * xv[k].re);
       const R ri = c[k].im - (res[k].im + s0 * xv[k].im);
       red[0] += (double)rr * rr + (double)ri * ri;
+      red[1] += (double)rr * pv[k].re + (double)ri * pv[k].im;  // <r, p> = sum conj(r) p
+      red[2] += (double)rr * pv[k].im - (double)ri * pv[k].re;
       lo[k] = mk<float>((float)rr, (float)ri);
     }
     rlo.template store<NCOMP>(s, lo);
   }
 };
+template <int P, int NCOMP>
+using EpiRu2 = EpiRu2Base<P, NCOMP, P32>;
+template <int P, int NCOMP>
+using EpiRu2H = EpiRu2Base<P, NCOMP, P16>;
 
 // ---------------------------------------------------------------------------
 // The kernel.  gs = site blocks; the grid has gs * nrhs blocks.
@@ -424,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_wilson(Lat lat, SiteMap map, SpinorV
 template <int P>
 struct TileView {
   using R = typename Store<P>::R;
-  using E = typename Store<P>::elem;
+  using E = typename Store<P>::lds_elem;  // decoded values (float2 for half storage)
   SpinorView<P> g;  // the field in global memory (hops leaving the tile)
   const E* lds;     // [NCOMP][nsite]
   int nsite, self;  // tile size, this thread's tile index
@@ -475,7 +484,7 @@ __global__ __launch_bounds__(P == P64 ? kTileMaxThreads64 : (HIF == 1 ? kTileMax
                                                                  LinkView<typename Store<P>::R> lkr, Epi epi,
                                                                  RedCtl rc, int nrhs) {
   using R = typename Store<P>::R;
-  using E = typename Store<P>::elem;
+  using E = typename Store<P>::lds_elem;
   constexpr int NCOMP = Gam<4>::NS * (ADJ ? NC * NC : NC);
   extern __shared__ __align__(16) unsigned char tile_smem[];
   int tblk = blockIdx.x, rhs = 0;
@@ -569,14 +578,14 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
   LinkView<R> lkr{(const cpx<R>*)lr, V, recon};
   SpinorView<P> sv{(typename Store<P>::elem*)in.data, in.norm, V};
   Epi epi(a, V);
-  if constexpr (D == 4 && P != P16 && !ADJ) {
+  if constexpr (D == 4 && !ADJ) {
     if (proj && wilson_tiled(d, P)) {
       const TileMap& tm = d.tile[P == P64 ? 0 : 1];
-      const size_t lds = (size_t)NCOMP * tm.nsite * sizeof(typename Store<P>::elem);
+      const size_t lds = (size_t)NCOMP * tm.nsite * sizeof(typename Store<P>::lds_elem);
       const dim3 grid((unsigned)tm.blocks * (unsigned)nrhs), block(tm.nsite);
       // hops in flight: 4 is built for complex64 only (complex128 would
       // spill); more than one needs blocks of at most 512 threads
-      int hif = (P == P64 && tm.hif == 4) ? 2 : tm.hif;
+      int hif = (P != P32 && tm.hif == 4) ? 2 : tm.hif;
       if (tm.nsite > 512) hif = 1;
       if (hif == 4) {
         if constexpr (P == P32)
@@ -669,11 +678,13 @@ int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, 
     case EPI_CG_K2X: return launch_prec<EpiCgK2X>(prec, d, proj, in, a, rc, stream, nrhs);
     case EPI_RU1:
     case EPI_RU2:
+    case EPI_RU2H:
       if (prec != P64) {
         set_error("the reliable update runs in complex128");
         return LYM_ERR_UNSUPPORTED;
       }
       if (kind == EPI_RU1) return launch_geom<P64, EpiRu1>(d, proj, in, a, rc, stream, nrhs);
+      if (kind == EPI_RU2H) return launch_geom<P64, EpiRu2H>(d, proj, in, a, rc, stream, nrhs);
       return launch_geom<P64, EpiRu2>(d, proj, in, a, rc, stream, nrhs);
     case EPI_MR_K1:
     case EPI_MR_K2:
