@@ -145,6 +145,7 @@ __device__ inline void cgm_fin_b_fast(SolverState* st, double rrn) {
 // <r_k, r_k> so the update can form beta from the true residual).
 __device__ inline void cg_fin_b_rel(SolverState* st, double rrn) {
   st->iter += 1;
+  st->x_pending = 1;  // x_lo += alpha p happens with the next p update (or the reliable update)
   if (!isfinite(rrn)) {
     st->status = ST_BREAKDOWN;
     st->phase = PH_DONE;
@@ -172,6 +173,7 @@ __device__ inline void cg_fin_b_rel(SolverState* st, double rrn) {
 // arXiv:0911.3191), which low-precision storage of p otherwise erodes.
 __device__ inline void cg_fin_ru(SolverState* st, double rr_true, double rp_re, double rp_im) {
   st->n_ru += 1;
+  st->x_pending = 0;  // folded into x_hi by the update
   if (!isfinite(rr_true)) {
     st->status = ST_BREAKDOWN;
     st->phase = PH_DONE;

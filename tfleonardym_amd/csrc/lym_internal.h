@@ -74,7 +74,7 @@ enum EpiKind : int {
   EPI_MR_K1 = 4,     // out = Op in (phase gated)
   EPI_MR_K2 = 5,     // out = Op in + sigma aux0; reduce <out,aux0>, <out,out>; MR finalizer
   EPI_CGM_K2F = 6,   // fast CG-M: ring[k+1] = ring[k] - alpha (Op in + sigma0 aux0); deferred finalizer
-  EPI_CG_K2X = 7,    // mixed CG: aux1 -= alpha (Op in + sigma0 aux0); out += alpha aux0; reduce |aux1|^2
+  EPI_CG_K2X = 7,    // mixed CG: aux1 -= alpha (Op in + sigma0 aux0); reduce |aux1|^2
   EPI_RU1 = 8,       // reliable update (complex128, gate RU): out = Op in
   EPI_RU2 = 9,       // reliable update: r = bs - (Op in + sigma0 aux0) -> out (complex64); reduce |r|^2
   EPI_RU2H = 10,     // the same with out in half storage (per-site norms)

@@ -74,8 +74,9 @@ struct SolverState {
   // reliable update, where it also removes the component of p along the new
   // residual (kp_a = 1 - beta <r,p>/<r,r>)
   double kp_a_re, kp_a_im;
-  int ru_final;  // the pending update follows the iterated stopping test
-  int n_ru;      // reliable updates done
+  int ru_final;   // the pending update follows the iterated stopping test
+  int n_ru;       // reliable updates done
+  int x_pending;  // x_lo += alpha p of the last step is still to be applied
 };
 
 }  // namespace lym

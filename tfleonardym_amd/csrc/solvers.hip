@@ -389,18 +389,49 @@ __global__ __launch_bounds__(kBlock) void k_rel_init(int64_t n, const Field* bs,
   block_partials<1>(red, partials);
 }
 
-// reliable update, accumulation (gate RU): x_hi += x_lo; x_lo = 0
-__global__ __launch_bounds__(kBlock) void k_ru_add(const SolverState* st, int64_t n, float2* __restrict__ x,
-                                                   double2* __restrict__ x64) {
+// direction update (gate KP): x_lo += alpha p (the last step's, if pending),
+// then p = kp_a r + beta p
+__global__ __launch_bounds__(kBlock) void k_rel_kp(const SolverState* st, int64_t n, const float2* __restrict__ r,
+                                                   float2* __restrict__ p, float2* __restrict__ x) {
   const int rhs = blockIdx.y;
-  if (st[rhs].phase != PH_RU) return;
+  st += rhs;
+  if (st->phase != PH_KP) return;
+  r += rhs * n;
+  p += rhs * n;
   x += rhs * n;
+  const float b = (float)st->beta_old, al = (float)st->alpha;
+  const float ar = (float)st->kp_a_re, ai = (float)st->kp_a_im;
+  const bool pend = st->x_pending != 0;
+  LYM_GRID_STRIDE(i, n) {
+    const float2 rv = r[i];
+    const float2 q = p[i];
+    if (pend) {
+      float2 xv = x[i];
+      xv.x += al * q.x;
+      xv.y += al * q.y;
+      x[i] = xv;
+    }
+    p[i] = make_float2(ar * rv.x - ai * rv.y + b * q.x, ar * rv.y + ai * rv.x + b * q.y);
+  }
+}
+
+// reliable update, accumulation (gate RU): x_hi += x_lo (+ alpha p if the
+// last step is pending); x_lo = 0
+__global__ __launch_bounds__(kBlock) void k_ru_add(const SolverState* st, int64_t n, float2* __restrict__ x,
+                                                   const float2* __restrict__ p, double2* __restrict__ x64) {
+  const int rhs = blockIdx.y;
+  st += rhs;
+  if (st->phase != PH_RU) return;
+  x += rhs * n;
+  p += rhs * n;
   x64 += rhs * n;
+  const double al = st->x_pending ? st->alpha : 0.0;
   LYM_GRID_STRIDE(i, n) {
     const float2 v = x[i];
+    const float2 q = p[i];
     double2 w = x64[i];
-    w.x += (double)v.x;
-    w.y += (double)v.y;
+    w.x += (double)v.x + al * (double)q.x;
+    w.y += (double)v.y + al * (double)q.y;
     x64[i] = w;
     x[i] = make_float2(0.f, 0.f);
   }
@@ -461,19 +492,27 @@ __global__ __launch_bounds__(kBlock) void k_rel_init16(int64_t V, const Field* b
   block_partials<1>(red, partials);
 }
 
-// p = r + beta p (gate KP)
+// x_lo += alpha p (pending), p = kp_a r + beta p (gate KP)
 template <int NCOMP>
-__global__ __launch_bounds__(kBlock) void k_kp16(const SolverState* st, int64_t V, Field r, Field p) {
+__global__ __launch_bounds__(kBlock) void k_kp16(const SolverState* st, int64_t V, Field r, Field p, Field x) {
   const int rhs = blockIdx.y;
   st += rhs;
   if (st->phase != PH_KP) return;
-  const SpinorView<P16> rv = half_view<NCOMP>(r, V, rhs), pv = half_view<NCOMP>(p, V, rhs);
-  const float b = (float)st->beta_old;
-  const float ar = (float)st->kp_a_re, ai = (float)st->kp_a_im;  // p = kp_a r + beta p
+  const SpinorView<P16> rv = half_view<NCOMP>(r, V, rhs), pv = half_view<NCOMP>(p, V, rhs),
+                        xv = half_view<NCOMP>(x, V, rhs);
+  const float b = (float)st->beta_old, al = (float)st->alpha;
+  const float ar = (float)st->kp_a_re, ai = (float)st->kp_a_im;
+  const bool pend = st->x_pending != 0;
   LYM_SITE_STRIDE(s, V) {
     cpx<float> a[NCOMP], q[NCOMP];
-    rv.template load<NCOMP>(s, a);
     pv.template load<NCOMP>(s, q);
+    if (pend) {
+      xv.template load<NCOMP>(s, a);
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) a[c] = mk<float>(a[c].re + al * q[c].re, a[c].im + al * q[c].im);
+      xv.template store<NCOMP>(s, a);
+    }
+    rv.template load<NCOMP>(s, a);
 #pragma unroll
     for (int c = 0; c < NCOMP; ++c)
       q[c] = mk<float>(ar * a[c].re - ai * a[c].im + b * q[c].re, ar * a[c].im + ai * a[c].re + b * q[c].im);
@@ -481,22 +520,27 @@ __global__ __launch_bounds__(kBlock) void k_kp16(const SolverState* st, int64_t 
   }
 }
 
-// reliable update, accumulation (gate RU): x_hi += x_lo; x_lo = 0
+// reliable update, accumulation (gate RU): x_hi += x_lo (+ alpha p if
+// pending); x_lo = 0
 template <int NCOMP>
-__global__ __launch_bounds__(kBlock) void k_ru_add16(const SolverState* st, int64_t V, Field x,
+__global__ __launch_bounds__(kBlock) void k_ru_add16(const SolverState* st, int64_t V, Field x, Field p,
                                                      double2* __restrict__ x64) {
   const int rhs = blockIdx.y;
-  if (st[rhs].phase != PH_RU) return;
-  const SpinorView<P16> xv = half_view<NCOMP>(x, V, rhs);
+  st += rhs;
+  if (st->phase != PH_RU) return;
+  const SpinorView<P16> xv = half_view<NCOMP>(x, V, rhs), pv = half_view<NCOMP>(p, V, rhs);
   const SpinorView<P64> xh{x64 + (int64_t)rhs * NCOMP * V, nullptr, V};
+  const double al = st->x_pending ? st->alpha : 0.0;
   LYM_SITE_STRIDE(s, V) {
-    cpx<float> a[NCOMP], z[NCOMP];
+    cpx<float> a[NCOMP], q[NCOMP], z[NCOMP];
     cpx<double> w[NCOMP];
     xv.template load<NCOMP>(s, a);
+    pv.template load<NCOMP>(s, q);
     xh.template load<NCOMP>(s, w);
 #pragma unroll
     for (int c = 0; c < NCOMP; ++c) {
-      w[c] = mk<double>(w[c].re + (double)a[c].re, w[c].im + (double)a[c].im);
+      w[c] = mk<double>(w[c].re + (double)a[c].re + al * (double)q[c].re,
+                        w[c].im + (double)a[c].im + al * (double)q[c].im);
       z[c] = czero<float>();
     }
     xh.template store<NCOMP>(s, w);
@@ -1012,10 +1056,11 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
 // stopping test), then x_hi += x_lo and r = b - (Q^2 + sigma) x_hi are formed
 // in FP64 and replace the iterated residual, p is kept.  Every RHS advances
 // in the same launches; one that has stopped skips them.  xs[r] (FP64) get
-// the solutions.  Iteration (each line one launch, RHS batched):
-//   p = r + beta p ; t = Q p (|t|^2, |p|^2) ; alpha ;
-//   r -= alpha (Q t + sigma p), x_lo += alpha p (|r|^2) ; beta / update test ;
-//   [update: x_hi += x_lo ; t = Q x_hi ; r = b - (Q t + sigma x_hi) (|r|^2) ; test]
+// the solutions.  Iteration (each step one launch, RHS batched):
+//   x_lo += alpha p, p = r + beta p ; t = Q p (|t|^2, |p|^2) ; alpha ;
+//   r -= alpha (Q t + sigma p) (|r|^2) ; beta / update test ;
+//   after every second iteration, for the RHS that asked for it:
+//   [x_hi += x_lo + alpha p ; t = Q x_hi ; r = b - (Q t + sigma x_hi) (|r|^2, <r,p>) ; test]
 constexpr int kRelChunk = 8;  // iterations per host chunk
 
 static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* const* bs, void* const* xs, int nrhs,
@@ -1071,19 +1116,22 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
   if (rc) return rc;
   const int gwl = wilson_blocks(d, pl), gw64 = wilson_blocks(d, P64);
+  // the update accumulation is mostly gated off: a small grid-stride grid
+  const int gfr = std::min(gf, 1024), gsr = std::min(gs, 1024);
+  static_assert(kRelChunk % 2 == 0, "the last iteration of a chunk must carry the update kernels");
   auto chunk = [&]() -> int {
     for (int j = 0; j < kRelChunk; ++j) {
-      // p = r + beta p (complex64: the fast CG-M kernel with a one-slot ring)
+      // x_lo += alpha p (last step), p = r + beta p
       int e = LYM_OK;
       if (pl == P16) {
         e = with_ncomp([&](auto nc) {
           hipLaunchKernelGGL(k_kp16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, P.st, V, P.rl_r,
-                             P.rl_p);
+                             P.rl_p, P.rl_x);
         });
         if (e) return e;
       } else {
-        hipLaunchKernelGGL(k_fast_kp<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
-                           (const float2*)P.rl_r.data, n, 1, (float2*)P.rl_p.data);
+        hipLaunchKernelGGL(k_rel_kp, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const float2*)P.rl_r.data,
+                           (float2*)P.rl_p.data, (float2*)P.rl_x.data);
         LYM_HIP(hipGetLastError());
       }
       EpiArgs a1;
@@ -1094,7 +1142,6 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
       if (!e) e = launch_fin(FK_CG_A, P.st, P.partials, gwl, 2, nullptr, st, nrhs, P.ctl);
       if (e) return e;
       EpiArgs a2;
-      a2.out = P.rl_x;
       a2.aux0 = P.rl_p;
       a2.aux1 = P.rl_r;
       a2.st = P.st;
@@ -1102,16 +1149,19 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
       e = wilson_launch(EPI_CG_K2X, d, pl, true, P.rl_t, a2, redctl(P), st, nrhs);
       if (!e) e = launch_fin(FK_CG_B, P.st, P.partials, gwl, 1, nullptr, st, nrhs, P.ctl);
       if (e) return e;
-      // reliable update: every kernel below returns at once for RHS not in PH_RU
+      // reliable update after every second iteration (an RHS that asks for
+      // one after an even iteration skips the next): every kernel below
+      // returns at once for RHS not in PH_RU, so their cost is the launch
+      if (j % 2 == 0) continue;
       if (pl == P16) {
         e = with_ncomp([&](auto nc) {
-          hipLaunchKernelGGL(k_ru_add16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, P.st, V, P.rl_x,
-                             (double2*)P.rl_x64.data);
+          hipLaunchKernelGGL(k_ru_add16<decltype(nc)::value>, dim3(gsr, nrhs), dim3(kBlock), 0, st, P.st, V,
+                             P.rl_x, P.rl_p, (double2*)P.rl_x64.data);
         });
         if (e) return e;
       } else {
-        hipLaunchKernelGGL(k_ru_add, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (float2*)P.rl_x.data,
-                           (double2*)P.rl_x64.data);
+        hipLaunchKernelGGL(k_ru_add, dim3(gfr, nrhs), dim3(kBlock), 0, st, P.st, n, (float2*)P.rl_x.data,
+                           (const float2*)P.rl_p.data, (double2*)P.rl_x64.data);
         LYM_HIP(hipGetLastError());
       }
       EpiArgs a3;

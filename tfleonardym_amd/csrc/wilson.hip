@@ -268,22 +268,21 @@ struct EpiMrK2 {
 };
 
 // Mixed CG step 2 (low precision): Ap = Q t + s0 p; r <- r - alpha Ap;
-// x_lo <- x_lo + alpha p; reduce |r|^2.
+// reduce |r|^2.  p is read only for a non-zero shift (x_lo += alpha p is done
+// by the next direction update, which reads p anyway).
 template <int P, int NCOMP>
 struct EpiCgK2X {
   using R = typename Store<P>::R;
   static constexpr int NRED = 1;
-  SpinorView<P> p, r, x;
+  SpinorView<P> p, r;
   SolverState* st;
   int64_t stride;
   EpiCgK2X(const EpiArgs& a, int64_t V)
-      : p(field_view<P>(a.aux0, V)), r(field_view<P>(a.aux1, V)), x(field_view<P>(a.out, V)), st(a.st),
-        stride(a.rhs_stride) {}
+      : p(field_view<P>(a.aux0, V)), r(field_view<P>(a.aux1, V)), st(a.st), stride(a.rhs_stride) {}
   __device__ EpiCgK2X at(int rhs) const {
     EpiCgK2X e = *this;
     e.p = rhs_view(p, rhs, stride);
     e.r = rhs_view(r, rhs, stride);
-    e.x = rhs_view(x, rhs, stride);
     e.st = st + rhs;
     return e;
   }
@@ -291,24 +290,22 @@ struct EpiCgK2X {
   template <class CV>
   __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
                         double* red) const {
-    cpx<R> c[NCOMP], res[NCOMP], pv[NCOMP], v[NCOMP];
+    cpx<R> c[NCOMP], res[NCOMP], v[NCOMP];
     in.template load<NCOMP>(s, c);
-    p.template load<NCOMP>(s, pv);
     wilson_finish<NCOMP, R>(lat, c, hop, res);
     const R s0 = (R)st->sigma0;
     const R al = (R)st->alpha;
+    if (s0 != R(0)) {
+      cpx<R> pv[NCOMP];
+      p.template load<NCOMP>(s, pv);
+#pragma unroll
+      for (int k = 0; k < NCOMP; ++k) res[k] = mk<R>(res[k].re + s0 * pv[k].re, res[k].im + s0 * pv[k].im);
+    }
     r.template load<NCOMP>(s, v);
 #pragma unroll
-    for (int k = 0; k < NCOMP; ++k) {
-      const R apr = res[k].re + s0 * pv[k].re, api = res[k].im + s0 * pv[k].im;
-      v[k] = mk<R>(v[k].re - al * apr, v[k].im - al * api);
-    }
+    for (int k = 0; k < NCOMP; ++k) v[k] = mk<R>(v[k].re - al * res[k].re, v[k].im - al * res[k].im);
     r.template store<NCOMP>(s, v);
     red[0] += norm2<NCOMP, R>(v);
-    x.template load<NCOMP>(s, v);
-#pragma unroll
-    for (int k = 0; k < NCOMP; ++k) v[k] = mk<R>(v[k].re + al * pv[k].re, v[k].im + al * pv[k].im);
-    x.template store<NCOMP>(s, v);
   }
 };
 
