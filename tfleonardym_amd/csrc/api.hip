@@ -119,44 +119,55 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   // Blocks of 256..512 sites are preferred when the lattice admits them
   // (smaller tiles re-read more, larger ones lose occupancy); ties go to the
   // deeper x2 extent (neighbours nearer in memory), then the smaller tile.
-  for (int pi = 0; pi < 2; ++pi) {
-    const int64_t es = pi == 0 ? 16 : 8;
+  // lds_per_site: LDS bytes a tile site needs; [lo, hi]: preferred tile sizes
+  auto choose_tile = [&](int64_t lds_per_site, int max_threads, int lo, int hi) {
     TileMap best{};
-    if (w->ndim == 4 && !o.adj && w->dims[0] % kNumXcd == 0) {
-      const int L0 = w->dims[0], L1 = w->dims[1], L2 = w->dims[2], T = w->dims[3];
-      const int slab = L0 / kNumXcd;
-      auto part = [](int a, int L) { return a == L ? 0.0 : 2.0 / a; };
-      for (int pass = 0; pass < 2 && !best.on; ++pass) {
-        double best_cost = 1e30;
-        for (int a0 = 1; a0 <= slab; ++a0) {
-          if (slab % a0) continue;
-          for (int a1 = 1; a1 <= L1; ++a1) {
-            if (L1 % a1) continue;
-            for (int k = 1; k <= L2; ++k) {
-              if (L2 % k) continue;
-              const int64_t ns = (int64_t)a0 * a1 * k * T;
-              if (ns % 64 || ns > (pi == 0 ? kTileMaxThreads64 : kTileMaxThreads)) continue;
-              if (ns * o.ncomp * es > kTileMaxLds) continue;
-              const bool forced = want[0] && a0 == want[0] && a1 == want[1] && k == want[2];
-              if (pass == 0 && !forced && (ns < 256 || ns > 512)) continue;
-              const double cost = forced ? -1.0 : 1.0 + part(a0, L0) + part(a1, L1) + part(k, L2);
-              const bool tie = std::fabs(cost - best_cost) < 1e-9;
-              if (cost < best_cost - 1e-9 || (tie && (k > best.k || (k == best.k && ns < best.nsite)))) {
-                best_cost = cost;
-                best = TileMap{1, a0, a1, k, T, slab, slab / a0, L1 / a1, L2 / k, (int)ns, (int)(V / ns)};
-              }
+    if (w->ndim != 4 || o.adj || w->dims[0] % kNumXcd) return best;
+    const int L0 = w->dims[0], L1 = w->dims[1], L2 = w->dims[2], T = w->dims[3];
+    const int slab = L0 / kNumXcd;
+    auto part = [](int a, int L) { return a == L ? 0.0 : 2.0 / a; };
+    for (int pass = 0; pass < 2 && !best.on; ++pass) {
+      double best_cost = 1e30;
+      for (int a0 = 1; a0 <= slab; ++a0) {
+        if (slab % a0) continue;
+        for (int a1 = 1; a1 <= L1; ++a1) {
+          if (L1 % a1) continue;
+          for (int k = 1; k <= L2; ++k) {
+            if (L2 % k) continue;
+            const int64_t ns = (int64_t)a0 * a1 * k * T;
+            if (ns % 64 || ns > max_threads) continue;
+            if (ns * lds_per_site > kTileMaxLds) continue;
+            const bool forced = want[0] && a0 == want[0] && a1 == want[1] && k == want[2];
+            if (pass == 0 && !forced && (ns < lo || ns > hi)) continue;
+            const double cost = forced ? -1.0 : 1.0 + part(a0, L0) + part(a1, L1) + part(k, L2);
+            const bool tie = std::fabs(cost - best_cost) < 1e-9;
+            if (cost < best_cost - 1e-9 || (tie && (k > best.k || (k == best.k && ns < best.nsite)))) {
+              best_cost = cost;
+              best = TileMap{1, a0, a1, k, T, slab, slab / a0, L1 / a1, L2 / k, (int)ns, (int)(V / ns)};
             }
           }
         }
       }
     }
     best.hif = 1;
+    return best;
+  };
+  for (int pi = 0; pi < 2; ++pi) {
+    const int64_t es = pi == 0 ? 16 : 8;
+    TileMap best = choose_tile(o.ncomp * es, pi == 0 ? kTileMaxThreads64 : kTileMaxThreads, 256, 512);
     if (const char* hf = std::getenv("LYM_HIF")) {
       const int h = std::atoi(hf);
       if (h == 2 || h == 4) best.hif = h;
     }
     o.tile[pi] = best;
   }
+  // multi-RHS tile (complex64): kMrRhs spinors per site in LDS.  Opt-in
+  // (LYM_MR=1 in the environment): kMrRhs spinor tiles leave LDS room for
+  // only a few waves per CU, and measured on MI355X that costs more than the
+  // saved link traffic (batched CG, 16^3x32 and 24^3x48: 8% and 17% slower).
+  o.tile_mr = choose_tile(o.ncomp * 8 * kMrRhs, kTileMaxThreads, 128, 256);
+  const char* mr = std::getenv("LYM_MR");
+  o.use_mr = (mr && mr[0] == '1') ? 1 : 0;
   *d = o;
   return LYM_OK;
 }

@@ -28,6 +28,8 @@ struct OpDesc {
   int wthreads = 0;  // plain Wilson block size (threads, multiple of 64)
   TileMap tile[2]{};  // LDS-tiled kernel per precision: [0] complex128, [1] complex64
   int use_tile = 1;   // 0 forces the plain kernel
+  TileMap tile_mr{};  // multi-RHS LDS tile (complex64, kMrRhs spinors per site)
+  int use_mr = 0;     // 1: batched complex64 launches use the multi-RHS kernel
   int link_su = 0;    // links are SU(nc): fundamental kernels rebuild the last row
   const void* links64 = nullptr;
   const void* linksr64 = nullptr;
@@ -86,7 +88,14 @@ inline bool wilson_tiled(const OpDesc& d, int prec) {
   // half storage decodes into the complex64 tile (same LDS footprint)
   return d.use_tile && (prec == P64 || prec == P32 || prec == P16) && d.tile[prec == P64 ? 0 : 1].on;
 }
-inline int wilson_blocks(const OpDesc& d, int prec) {
+// batched complex64 launches on 4D fundamental lattices: kMrRhs RHS per block
+// share every link load
+constexpr int kMrRhs = 3;
+inline bool wilson_mr(const OpDesc& d, int prec, int nrhs) {
+  return d.use_tile && d.use_mr && prec == P32 && nrhs > 1 && d.tile_mr.on;
+}
+inline int wilson_blocks(const OpDesc& d, int prec, int nrhs = 1) {
+  if (wilson_mr(d, prec, nrhs)) return d.tile_mr.blocks;
   return wilson_tiled(d, prec) ? d.tile[prec == P64 ? 0 : 1].blocks : d.wblocks;
 }
 

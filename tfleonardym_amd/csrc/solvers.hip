@@ -365,110 +365,29 @@ __global__ __launch_bounds__(kBlock) void k_fast_switch(SolverState* st, int64_t
   }
 }
 
-// ---- mixed CG with reliable updates (grid.y = RHS) -------------------------
-// init: r_lo = b, p_lo = 0, x_lo = 0, x_hi = 0; partial |b|^2 (FP64)
-__global__ __launch_bounds__(kBlock) void k_rel_init(int64_t n, const Field* bs, float2* __restrict__ r,
-                                                     float2* __restrict__ p, float2* __restrict__ x,
-                                                     double2* __restrict__ x64, double* partials) {
-  const int rhs = blockIdx.y;
-  const double2* __restrict__ b = (const double2*)bs[rhs].data;
-  r += rhs * n;
-  p += rhs * n;
-  x += rhs * n;
-  x64 += rhs * n;
-  partials += (int64_t)rhs * gridDim.x;
-  double red[1] = {0.0};
-  LYM_GRID_STRIDE(i, n) {
-    const double2 v = b[i];
-    red[0] += v.x * v.x + v.y * v.y;
-    r[i] = make_float2((float)v.x, (float)v.y);
-    p[i] = make_float2(0.f, 0.f);
-    x[i] = make_float2(0.f, 0.f);
-    x64[i] = make_double2(0.0, 0.0);
-  }
-  block_partials<1>(red, partials);
-}
-
-// direction update (gate KP): x_lo += alpha p (the last step's, if pending),
-// then p = kp_a r + beta p
-__global__ __launch_bounds__(kBlock) void k_rel_kp(const SolverState* st, int64_t n, const float2* __restrict__ r,
-                                                   float2* __restrict__ p, float2* __restrict__ x) {
-  const int rhs = blockIdx.y;
-  st += rhs;
-  if (st->phase != PH_KP) return;
-  r += rhs * n;
-  p += rhs * n;
-  x += rhs * n;
-  const float b = (float)st->beta_old, al = (float)st->alpha;
-  const float ar = (float)st->kp_a_re, ai = (float)st->kp_a_im;
-  const bool pend = st->x_pending != 0;
-  LYM_GRID_STRIDE(i, n) {
-    const float2 rv = r[i];
-    const float2 q = p[i];
-    if (pend) {
-      float2 xv = x[i];
-      xv.x += al * q.x;
-      xv.y += al * q.y;
-      x[i] = xv;
-    }
-    p[i] = make_float2(ar * rv.x - ai * rv.y + b * q.x, ar * rv.y + ai * rv.x + b * q.y);
-  }
-}
-
-// reliable update, accumulation (gate RU): x_hi += x_lo (+ alpha p if the
-// last step is pending); x_lo = 0
-__global__ __launch_bounds__(kBlock) void k_ru_add(const SolverState* st, int64_t n, float2* __restrict__ x,
-                                                   const float2* __restrict__ p, double2* __restrict__ x64) {
-  const int rhs = blockIdx.y;
-  st += rhs;
-  if (st->phase != PH_RU) return;
-  x += rhs * n;
-  p += rhs * n;
-  x64 += rhs * n;
-  const double al = st->x_pending ? st->alpha : 0.0;
-  LYM_GRID_STRIDE(i, n) {
-    const float2 v = x[i];
-    const float2 q = p[i];
-    double2 w = x64[i];
-    w.x += (double)v.x + al * (double)q.x;
-    w.y += (double)v.y + al * (double)q.y;
-    x64[i] = w;
-    x[i] = make_float2(0.f, 0.f);
-  }
-}
-
-// result: xs[rhs] = x_hi + x_lo (x_lo is zero after a final update)
-__global__ __launch_bounds__(kBlock) void k_rel_out(int64_t n, const float2* __restrict__ x,
-                                                    const double2* __restrict__ x64, const Field* xs) {
-  const int rhs = blockIdx.y;
-  double2* __restrict__ out = (double2*)xs[rhs].data;
-  x += rhs * n;
-  x64 += rhs * n;
-  LYM_GRID_STRIDE(i, n) {
-    const float2 v = x[i];
-    const double2 w = x64[i];
-    out[i] = make_double2(w.x + (double)v.x, w.y + (double)v.y);
-  }
-}
-
-// ---- the same steps with half storage (per-site norms): a thread owns whole
-// sites, since storing a site rescales all its components.  Fields of RHS i
-// start at data + i * NCOMP * V and norm + i * V.
+// ---- mixed CG with reliable updates: site kernels (grid.y = RHS) -----------
+// A thread owns whole sites.  r, p and the solution since the last update,
+// x_lo, are stored at precision PV (complex64, or 16-bit fixed point with
+// per-site norms).  (16-bit x_lo under complex64 r, p was measured: point
+// sources on 24^3x48 then need 2.8x the iterations.)
+// Fields of RHS i start at data + i * NCOMP * V (and norm + i * V).
 #define LYM_SITE_STRIDE(s, V) \
   for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < (V); s += (int64_t)gridDim.x * kBlock)
 
-template <int NCOMP>
-__device__ __forceinline__ SpinorView<P16> half_view(const Field& f, int64_t V, int rhs) {
-  return SpinorView<P16>{(Store<P16>::elem*)f.data + (int64_t)rhs * NCOMP * V, f.norm + (int64_t)rhs * V, V};
+template <int PV, int NCOMP>
+__device__ __forceinline__ SpinorView<PV> vec_view(const Field& f, int64_t V, int rhs) {
+  return SpinorView<PV>{(typename Store<PV>::elem*)f.data + (int64_t)rhs * NCOMP * V,
+                        f.norm ? f.norm + (int64_t)rhs * V : nullptr, V};
 }
 
-template <int NCOMP>
-__global__ __launch_bounds__(kBlock) void k_rel_init16(int64_t V, const Field* bs, Field r, Field p, Field x,
-                                                       double2* __restrict__ x64, double* partials) {
+// init: r = b, p = 0, x_lo = 0, x_hi = 0; partial |b|^2 (FP64)
+template <int PV, int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_rel_init(int64_t V, const Field* bs, Field r, Field p, Field x,
+                                                     double2* __restrict__ x64, double* partials) {
   const int rhs = blockIdx.y;
   const SpinorView<P64> b{(double2*)bs[rhs].data, nullptr, V};
-  const SpinorView<P16> rv = half_view<NCOMP>(r, V, rhs), pv = half_view<NCOMP>(p, V, rhs),
-                        xv = half_view<NCOMP>(x, V, rhs);
+  const SpinorView<PV> rv = vec_view<PV, NCOMP>(r, V, rhs), pv = vec_view<PV, NCOMP>(p, V, rhs);
+  const SpinorView<PV> xv = vec_view<PV, NCOMP>(x, V, rhs);
   const SpinorView<P64> xh{x64 + (int64_t)rhs * NCOMP * V, nullptr, V};
   partials += (int64_t)rhs * gridDim.x;
   double red[1] = {0.0};
@@ -492,14 +411,15 @@ __global__ __launch_bounds__(kBlock) void k_rel_init16(int64_t V, const Field* b
   block_partials<1>(red, partials);
 }
 
-// x_lo += alpha p (pending), p = kp_a r + beta p (gate KP)
-template <int NCOMP>
-__global__ __launch_bounds__(kBlock) void k_kp16(const SolverState* st, int64_t V, Field r, Field p, Field x) {
+// direction update (gate KP): x_lo += alpha p (the last step's, if pending),
+// then p = kp_a r + beta p
+template <int PV, int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_rel_kp(const SolverState* st, int64_t V, Field r, Field p, Field x) {
   const int rhs = blockIdx.y;
   st += rhs;
   if (st->phase != PH_KP) return;
-  const SpinorView<P16> rv = half_view<NCOMP>(r, V, rhs), pv = half_view<NCOMP>(p, V, rhs),
-                        xv = half_view<NCOMP>(x, V, rhs);
+  const SpinorView<PV> rv = vec_view<PV, NCOMP>(r, V, rhs), pv = vec_view<PV, NCOMP>(p, V, rhs);
+  const SpinorView<PV> xv = vec_view<PV, NCOMP>(x, V, rhs);
   const float b = (float)st->beta_old, al = (float)st->alpha;
   const float ar = (float)st->kp_a_re, ai = (float)st->kp_a_im;
   const bool pend = st->x_pending != 0;
@@ -520,15 +440,16 @@ __global__ __launch_bounds__(kBlock) void k_kp16(const SolverState* st, int64_t 
   }
 }
 
-// reliable update, accumulation (gate RU): x_hi += x_lo (+ alpha p if
-// pending); x_lo = 0
-template <int NCOMP>
-__global__ __launch_bounds__(kBlock) void k_ru_add16(const SolverState* st, int64_t V, Field x, Field p,
-                                                     double2* __restrict__ x64) {
+// reliable update, accumulation (gate RU): x_hi += x_lo (+ alpha p if the
+// last step is pending); x_lo = 0
+template <int PV, int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_ru_add(const SolverState* st, int64_t V, Field x, Field p,
+                                                   double2* __restrict__ x64) {
   const int rhs = blockIdx.y;
   st += rhs;
   if (st->phase != PH_RU) return;
-  const SpinorView<P16> xv = half_view<NCOMP>(x, V, rhs), pv = half_view<NCOMP>(p, V, rhs);
+  const SpinorView<PV> xv = vec_view<PV, NCOMP>(x, V, rhs);
+  const SpinorView<PV> pv = vec_view<PV, NCOMP>(p, V, rhs);
   const SpinorView<P64> xh{x64 + (int64_t)rhs * NCOMP * V, nullptr, V};
   const double al = st->x_pending ? st->alpha : 0.0;
   LYM_SITE_STRIDE(s, V) {
@@ -548,12 +469,12 @@ __global__ __launch_bounds__(kBlock) void k_ru_add16(const SolverState* st, int6
   }
 }
 
-// result: xs[rhs] = x_hi + x_lo
-template <int NCOMP>
-__global__ __launch_bounds__(kBlock) void k_rel_out16(int64_t V, Field x, const double2* __restrict__ x64,
-                                                      const Field* xs) {
+// result: xs[rhs] = x_hi + x_lo (x_lo is zero after a final update)
+template <int PV, int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_rel_out(int64_t V, Field x, const double2* __restrict__ x64,
+                                                    const Field* xs) {
   const int rhs = blockIdx.y;
-  const SpinorView<P16> xv = half_view<NCOMP>(x, V, rhs);
+  const SpinorView<PV> xv = vec_view<PV, NCOMP>(x, V, rhs);
   const SpinorView<P64> xh{const_cast<double2*>(x64) + (int64_t)rhs * NCOMP * V, nullptr, V};
   const SpinorView<P64> out{(double2*)xs[rhs].data, nullptr, V};
   LYM_SITE_STRIDE(s, V) {
@@ -684,7 +605,8 @@ struct Parts {
   Field tb64, tb32;         // t = Q p
   Field ps64;               // nrhs x nshift FP64 shift directions
   // mixed CG with reliable updates (one shift), every field nrhs x batched:
-  Field rl_r, rl_p, rl_t, rl_x;  // complex64 residual, direction, Q p, solution since the last update
+  // residual, direction, Q p, solution since the last update (complex64 or 16-bit storage)
+  Field rl_r, rl_p, rl_t, rl_x;
   Field rl_x64, rl_t64;          // complex128 accumulated solution, Q x
 };
 
@@ -967,7 +889,7 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
     const Field ring = prec == P64 ? P.ring64 : P.ring32;
     const Field pl = prec == P64 ? P.plive64 : P.plive32;
     const Field t = prec == P64 ? P.tb64 : P.tb32;
-    const int gw = wilson_blocks(d, prec);  // Wilson partials per RHS at this precision
+    const int gw = wilson_blocks(d, prec, nrhs);  // Wilson partials per RHS at this precision
     const int mb = prec == P64 ? kFastBlock64 : kFastBlock32;
     const int nslot = prec == P64 ? nslot64 : nslot32;
     const int64_t ring_stride = prec == P64 ? ring_stride64 : ring_stride32;
@@ -1065,19 +987,25 @@ constexpr int kRelChunk = 8;  // iterations per host chunk
 
 static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* const* bs, void* const* xs, int nrhs,
                        double sigma, double tol, int max_iter, hipStream_t st, std::vector<SolverState>& out_states) {
-  // half storage: site kernels instantiated per component count
-  auto with_ncomp = [&](auto f) -> int {
-    switch (d.ncomp) {
-      case 4: f(IC<4>{}); break;
-      case 6: f(IC<6>{}); break;
-      case 8: f(IC<8>{}); break;
-      case 12: f(IC<12>{}); break;
-      default:
-        set_error("half-precision storage: fundamental representation only");
-        return LYM_ERR_UNSUPPORTED;
-    }
-    LYM_HIP(hipGetLastError());
-    return LYM_OK;
+  // site kernels instantiated per storage precision of r, p and component count
+  auto dispatch = [&](auto f) -> int {
+    auto by_nc = [&](auto pv) -> int {
+      switch (d.ncomp) {
+        case 4: f(pv, IC<4>{}); break;
+        case 6: f(pv, IC<6>{}); break;
+        case 8: f(pv, IC<8>{}); break;
+        case 12: f(pv, IC<12>{}); break;
+        case 16: f(pv, IC<16>{}); break;
+        case 18: f(pv, IC<18>{}); break;
+        case 36: f(pv, IC<36>{}); break;
+        default:
+          set_error("mixed CG: unsupported number of spinor components");
+          return LYM_ERR_UNSUPPORTED;
+      }
+      LYM_HIP(hipGetLastError());
+      return LYM_OK;
+    };
+    return pl == P16 ? by_nc(IC<P16>{}) : by_nc(IC<P32>{});
   };
   const int64_t V = d.lat.V;
   const int gs = flat_grid(V);
@@ -1100,40 +1028,25 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   LYM_HIP(hipMemcpyAsync(P.xs_dev, hx.data(), sizeof(Field) * nrhs, hipMemcpyHostToDevice, st));
   const int64_t n = field_elems(d);
   const int gr = red_grid_flat(n);
-  const int gf = flat_grid(n);
-  int rc = LYM_OK;
-  if (pl == P16) {
-    rc = with_ncomp([&](auto nc) {
-      hipLaunchKernelGGL(k_rel_init16<decltype(nc)::value>, dim3(gr, nrhs), dim3(kBlock), 0, st, V, P.bs_dev,
-                         P.rl_r, P.rl_p, P.rl_x, (double2*)P.rl_x64.data, P.partials);
-    });
-  } else {
-    hipLaunchKernelGGL(k_rel_init, dim3(gr, nrhs), dim3(kBlock), 0, st, n, P.bs_dev, (float2*)P.rl_r.data,
-                       (float2*)P.rl_p.data, (float2*)P.rl_x.data, (double2*)P.rl_x64.data, P.partials);
-    LYM_HIP(hipGetLastError());
-  }
+  int rc = dispatch([&](auto pv, auto nc) {
+    hipLaunchKernelGGL((k_rel_init<decltype(pv)::value, decltype(nc)::value>), dim3(gr, nrhs), dim3(kBlock), 0, st,
+                       V, P.bs_dev, P.rl_r, P.rl_p, P.rl_x, (double2*)P.rl_x64.data, P.partials);
+  });
   if (rc) return rc;
   rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
   if (rc) return rc;
-  const int gwl = wilson_blocks(d, pl), gw64 = wilson_blocks(d, P64);
+  const int gwl = wilson_blocks(d, pl, nrhs), gw64 = wilson_blocks(d, P64, nrhs);
   // the update accumulation is mostly gated off: a small grid-stride grid
-  const int gfr = std::min(gf, 1024), gsr = std::min(gs, 1024);
+  const int gsr = std::min(gs, 1024);
   static_assert(kRelChunk % 2 == 0, "the last iteration of a chunk must carry the update kernels");
   auto chunk = [&]() -> int {
     for (int j = 0; j < kRelChunk; ++j) {
       // x_lo += alpha p (last step), p = r + beta p
-      int e = LYM_OK;
-      if (pl == P16) {
-        e = with_ncomp([&](auto nc) {
-          hipLaunchKernelGGL(k_kp16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, P.st, V, P.rl_r,
-                             P.rl_p, P.rl_x);
-        });
-        if (e) return e;
-      } else {
-        hipLaunchKernelGGL(k_rel_kp, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const float2*)P.rl_r.data,
-                           (float2*)P.rl_p.data, (float2*)P.rl_x.data);
-        LYM_HIP(hipGetLastError());
-      }
+      int e = dispatch([&](auto pv, auto nc) {
+        hipLaunchKernelGGL((k_rel_kp<decltype(pv)::value, decltype(nc)::value>), dim3(gs, nrhs), dim3(kBlock), 0,
+                           st, P.st, V, P.rl_r, P.rl_p, P.rl_x);
+      });
+      if (e) return e;
       EpiArgs a1;
       a1.out = P.rl_t;
       a1.st = P.st;
@@ -1153,17 +1066,11 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
       // one after an even iteration skips the next): every kernel below
       // returns at once for RHS not in PH_RU, so their cost is the launch
       if (j % 2 == 0) continue;
-      if (pl == P16) {
-        e = with_ncomp([&](auto nc) {
-          hipLaunchKernelGGL(k_ru_add16<decltype(nc)::value>, dim3(gsr, nrhs), dim3(kBlock), 0, st, P.st, V,
-                             P.rl_x, P.rl_p, (double2*)P.rl_x64.data);
-        });
-        if (e) return e;
-      } else {
-        hipLaunchKernelGGL(k_ru_add, dim3(gfr, nrhs), dim3(kBlock), 0, st, P.st, n, (float2*)P.rl_x.data,
-                           (const float2*)P.rl_p.data, (double2*)P.rl_x64.data);
-        LYM_HIP(hipGetLastError());
-      }
+      e = dispatch([&](auto pv, auto nc) {
+        hipLaunchKernelGGL((k_ru_add<decltype(pv)::value, decltype(nc)::value>), dim3(gsr, nrhs), dim3(kBlock), 0,
+                           st, P.st, V, P.rl_x, P.rl_p, (double2*)P.rl_x64.data);
+      });
+      if (e) return e;
       EpiArgs a3;
       a3.out = P.rl_t64;
       a3.st = P.st;
@@ -1209,17 +1116,12 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
     }
     if ((rc = issue(slot))) return rc;
   }
-  if (pl == P16) {
-    rc = with_ncomp([&](auto nc) {
-      hipLaunchKernelGGL(k_rel_out16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, V, P.rl_x,
-                         (const double2*)P.rl_x64.data, P.xs_dev);
-    });
-    if (rc) return rc;
-  } else {
-    hipLaunchKernelGGL(k_rel_out, dim3(gf, nrhs), dim3(kBlock), 0, st, n, (const float2*)P.rl_x.data,
+  rc = dispatch([&](auto pv, auto nc) {
+    hipLaunchKernelGGL((k_rel_out<decltype(pv)::value, decltype(nc)::value>), dim3(gs, nrhs), dim3(kBlock), 0, st, V,
+                       P.rl_x,
                        (const double2*)P.rl_x64.data, P.xs_dev);
-    LYM_HIP(hipGetLastError());
-  }
+  });
+  if (rc) return rc;
   out_states.resize(nrhs);
   LYM_HIP(hipMemcpyAsync(out_states.data(), P.st, sizeof(SolverState) * nrhs, hipMemcpyDeviceToHost, st));
   LYM_HIP(hipStreamSynchronize(st));

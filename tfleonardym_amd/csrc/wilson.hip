@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "lym_fin.h"
 #include "lym_internal.h"
@@ -555,6 +556,150 @@ __global__ __launch_bounds__(P == P64 ? kTileMaxThreads64 : (HIF == 1 ? kTileMax
 }
 
 // ---------------------------------------------------------------------------
+// Multi-RHS LDS-tiled kernel (4D fundamental, complex64): a block owns a tile
+// and a group of kMrRhs consecutive right-hand sides, stages all of their
+// spinors in LDS, and applies every link it loads to each of them, so link
+// traffic per RHS drops by the group size.  Per RHS the arithmetic is the
+// single-RHS kernel's (same order), hence the same result.  Blocks of one
+// tile's groups run back to back on one XCD.
+// ---------------------------------------------------------------------------
+template <class T, int N>
+struct Arr {
+  T v[N];
+};
+
+template <int NC, int P, class Epi, size_t... I>
+__device__ __forceinline__ Arr<Epi, sizeof...(I)> group_epis(const Epi& epi, int rhs0, int nrhs,
+                                                             std::index_sequence<I...>) {
+  return Arr<Epi, sizeof...(I)>{{epi.at(min(rhs0 + (int)I, nrhs - 1))...}};
+}
+
+template <int NC, int P, class Epi>
+__global__ __launch_bounds__(kTileMaxThreads, 2) void k_wilson_tile_mr(Lat lat, TileMap tm, SpinorView<P> in,
+                                                                       int64_t in_stride,
+                                                                       LinkView<typename Store<P>::R> lk, Epi epi,
+                                                                       RedCtl rc, int nrhs, int ngroups) {
+  using R = typename Store<P>::R;
+  using E = typename Store<P>::lds_elem;
+  constexpr int NCOMP = Gam<4>::NS * NC;
+  constexpr int RB = kMrRhs;
+  extern __shared__ __align__(16) unsigned char tile_smem[];
+  const int b = blockIdx.x;
+  const int bx = b % kNumXcd;
+  const int jj = b / kNumXcd;
+  const int grp = jj % ngroups;
+  const int tblk = (jj / ngroups) * kNumXcd + bx;
+  const int rhs0 = grp * RB;
+  const Arr<Epi, RB> e = group_epis<NC, P>(epi, rhs0, nrhs, std::make_index_sequence<RB>{});
+  bool act[RB];
+  bool any = false;
+#pragma unroll
+  for (int r = 0; r < RB; ++r) {
+    act[r] = (rhs0 + r < nrhs) && e.v[r].active();  // uniform per block
+    any = any || act[r];
+  }
+  if (!any) return;
+  // tile origin and thread coordinates as in k_wilson_tile
+  const int xcd = tblk % kNumXcd;
+  const int j = tblk / kNumXcd;
+  const int i0 = j % tm.n0;
+  const int rr = j / tm.n0;
+  const int i2 = rr % tm.n2;
+  const int i1 = rr / tm.n2;
+  const int tid = (int)threadIdx.x;
+  int q = tid / tm.T;
+  const int t3 = tid - q * tm.T;
+  const int t2 = q % tm.k;
+  q /= tm.k;
+  const int t1 = q % tm.a1;
+  const int t0 = q / tm.a1;
+  int x[4];
+  x[0] = xcd * tm.slab + i0 * tm.a0 + t0;
+  x[1] = i1 * tm.a1 + t1;
+  x[2] = i2 * tm.k + t2;
+  x[3] = t3;
+  const int64_t s = (((int64_t)x[0] * lat.L[1] + x[1]) * lat.L[2] + x[2]) * tm.T + x[3];
+  E* lds = reinterpret_cast<E*>(tile_smem);
+  TileView<P> view[RB];
+#pragma unroll
+  for (int r = 0; r < RB; ++r) {
+    view[r].g = rhs_view(in, min(rhs0 + r, nrhs - 1), in_stride);
+    view[r].lds = lds + (int64_t)r * NCOMP * tm.nsite;
+    view[r].nsite = tm.nsite;
+    view[r].self = tid;
+    view[r].t[0] = t0;
+    view[r].t[1] = t1;
+    view[r].t[2] = t2;
+    view[r].t[3] = t3;
+    view[r].a[0] = tm.a0;
+    view[r].a[1] = tm.a1;
+    view[r].a[2] = tm.k;
+    view[r].a[3] = tm.T;
+    view[r].str[3] = 1;
+    view[r].str[2] = tm.T;
+    view[r].str[1] = tm.k * tm.T;
+    view[r].str[0] = tm.a1 * tm.k * tm.T;
+    view[r].full[0] = tm.a0 == lat.L[0];
+    view[r].full[1] = tm.a1 == lat.L[1];
+    view[r].full[2] = tm.k == lat.L[2];
+    view[r].full[3] = true;
+    if (act[r]) {
+      cpx<R> c[NCOMP];
+      view[r].g.template load<NCOMP>(s, c);
+#pragma unroll
+      for (int k = 0; k < NCOMP; ++k) {
+        E v;
+        v.x = c[k].re;
+        v.y = c[k].im;
+        lds[((int64_t)r * NCOMP + k) * tm.nsite + tid] = v;
+      }
+    }
+  }
+  __syncthreads();
+  cpx<R> acc[RB][NCOMP];
+#pragma unroll
+  for (int r = 0; r < RB; ++r)
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) acc[r][c] = czero<R>();
+#pragma unroll
+  for (int h = 0; h < 8; ++h) {
+    const int mu = h >> 1, dir = h & 1;
+    const int L = lat.L[mu];
+    const int64_t st = lat.stride[mu];
+    int64_t nb = dir == 0 ? ((x[mu] == L - 1) ? s - (int64_t)(L - 1) * st : s + st)
+                          : ((x[mu] == 0) ? s + (int64_t)(L - 1) * st : s - st);
+    int64_t ls = dir == 0 ? s : nb;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) hop_commit<NCOMP, R>(acc[r]);
+    hop_dep<R>(nb, ls, acc[0][0].re);
+    cpx<R> u[NC][NC];
+    lk.template load<NC>(mu, ls, u);
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      if (act[r]) {
+        cpx<R> v[NCOMP];
+        view[r].template load_hop<NCOMP>(mu, dir, nb, v);
+        wilson_hop_apply<4, NC, false, true, R>(lat, mu, dir, x[mu], v, u, u, acc[r]);
+      }
+    }
+  }
+  constexpr int NR = Epi::NRED > 0 ? Epi::NRED : 1;
+#pragma unroll
+  for (int r = 0; r < RB; ++r) {
+    if (!act[r]) continue;
+    double red[NR];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) red[i] = 0.0;
+    e.v[r].apply(lat, s, view[r], acc[r], red);
+    if constexpr (Epi::NRED > 0) {
+      block_partials<NR, kTileMaxThreads / 64>(red, rc.partials + (int64_t)(rhs0 + r) * NR * tm.blocks, tblk,
+                                               tm.blocks);
+      __syncthreads();  // the reduction's LDS is reused by the next RHS
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Dispatch: runtime (kind, ndim, nc, rep, prec, mode) -> template instance.
 // ---------------------------------------------------------------------------
 template <int D, int NC, bool ADJ, int P, template <int, int> class EpiT>
@@ -575,6 +720,18 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
   LinkView<R> lkr{(const cpx<R>*)lr, V, recon};
   SpinorView<P> sv{(typename Store<P>::elem*)in.data, in.norm, V};
   Epi epi(a, V);
+  if constexpr (D == 4 && !ADJ && P == P32) {
+    if (proj && wilson_mr(d, P, nrhs)) {
+      const TileMap& tm = d.tile_mr;
+      const int ngroups = (nrhs + kMrRhs - 1) / kMrRhs;
+      const size_t lds = (size_t)kMrRhs * NCOMP * tm.nsite * sizeof(typename Store<P>::lds_elem);
+      const dim3 grid((unsigned)tm.blocks * (unsigned)ngroups), block(tm.nsite);
+      hipLaunchKernelGGL((k_wilson_tile_mr<NC, P, Epi>), grid, block, lds, stream, d.lat, tm, sv, a.rhs_stride, lk,
+                         epi, rc, nrhs, ngroups);
+      LYM_HIP(hipGetLastError());
+      return LYM_OK;
+    }
+  }
   if constexpr (D == 4 && !ADJ) {
     if (proj && wilson_tiled(d, P)) {
       const TileMap& tm = d.tile[P == P64 ? 0 : 1];

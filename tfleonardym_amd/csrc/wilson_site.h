@@ -133,26 +133,19 @@ __host__ __device__ __forceinline__ void transport(const cpx<R> (&u)[NC][NC], co
 // (without kappa, without the identity term, without gamma5).
 //   PROJ = true : half-spinor projection + reconstruction (dirac.py:458-501)
 //   PROJ = false: full projector on all Ns rows (dirac.py:503-523)
-// SV: spinor view with load<NS*NCC>(site, v); links/links_r: LinkView<R>.
-// One hop (mu, dir: 0 = +mu, 1 = -mu) of a site with coordinate xmu along
-// mu: neighbour spinor at site nb, link at site ls (forward: U_mu(x);
-// backward: U_mu(x - mu)), accumulated into acc.
-template <int D, int NC, bool ADJ, bool PROJ, typename R, typename SV>
-__host__ __device__ __forceinline__ void wilson_hop_one(const Lat& lat, int mu, int dir, int xmu, int64_t nb,
-                                                        int64_t ls, const SV& psi, const LinkView<R>& links,
-                                                        const LinkView<R>& links_r,
-                                                        cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
+// One hop term (mu, dir: 0 = +mu, 1 = -mu) of a site with coordinate xmu
+// along mu, given the neighbour spinor v and the hop's link(s) u, ur
+// (forward: U_mu(x); backward: U_mu(x - mu)), accumulated into acc.
+template <int D, int NC, bool ADJ, bool PROJ, typename R>
+__host__ __device__ __forceinline__ void wilson_hop_apply(const Lat& lat, int mu, int dir, int xmu,
+                                                          const cpx<R> (&v)[Gam<D>::NS * (ADJ ? NC * NC : NC)],
+                                                          const cpx<R> (&u)[NC][NC], const cpx<R> (&ur)[NC][NC],
+                                                          cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
   constexpr int NS = Gam<D>::NS;
   constexpr int H = NS / 2;
   constexpr int NCC = ADJ ? NC * NC : NC;
   constexpr int NCOMP = NS * NCC;
   const bool fwd = (dir == 0);
-  cpx<R> v[NCOMP];
-  psi.template load_hop<NCOMP>(mu, dir, nb, v);
-  cpx<R> u[NC][NC];
-  links.template load<NC>(mu, ls, u);
-  cpx<R> ur[NC][NC];
-  if constexpr (ADJ) links_r.template load<NC>(mu, ls, ur);
   // forward term uses (1 - g), backward (1 + g): sg = -1 / +1
   const R sg = fwd ? R(-1) : R(1);
   const bool edge = lat.has_bc && (mu == D - 1) && (fwd ? xmu == lat.L[mu] - 1 : xmu == 0);
@@ -213,6 +206,23 @@ __host__ __device__ __forceinline__ void wilson_hop_one(const Lat& lat, int mu, 
       for (int c = 0; c < NCC; ++c) acc[sp * NCC + c] += chi[c];
     }
   }
+}
+
+// One hop: neighbour spinor at site nb (psi.load_hop), link(s) at site ls.
+// SV: spinor view with load_hop<NS*NCC>(mu, dir, site, v); links: LinkView<R>.
+template <int D, int NC, bool ADJ, bool PROJ, typename R, typename SV>
+__host__ __device__ __forceinline__ void wilson_hop_one(const Lat& lat, int mu, int dir, int xmu, int64_t nb,
+                                                        int64_t ls, const SV& psi, const LinkView<R>& links,
+                                                        const LinkView<R>& links_r,
+                                                        cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
+  constexpr int NCOMP = Gam<D>::NS * (ADJ ? NC * NC : NC);
+  cpx<R> v[NCOMP];
+  psi.template load_hop<NCOMP>(mu, dir, nb, v);
+  cpx<R> u[NC][NC];
+  links.template load<NC>(mu, ls, u);
+  cpx<R> ur[NC][NC];
+  if constexpr (ADJ) links_r.template load<NC>(mu, ls, ur);
+  wilson_hop_apply<D, NC, ADJ, PROJ, R>(lat, mu, dir, xmu, v, u, ur, acc);
 }
 
 // Hops in the order (mu, +), (mu, -), mu = 0..D-1, in groups of HIF: the
