@@ -190,17 +190,119 @@ def bytes_per_site_dslash(nc, ndim, wl, ws, link_rows=None):
     return ndim * rows * nc * 2 * wl + 2 * ns * nc * 2 * ws
 
 
+# D at 24^3 x 48 SU(3) (the batched-solve lattice): roofline fractions are
+# quoted here, where the working set does not fit the 256 MB MALL
+DSLASH_C3 = dict(geometry=(24, 24, 24, 48), nc=3, kappa=0.1335, eps=0.25, seed=20250)
+
+
+def time_dslash(op, src, reps):
+    """Average launch time of D (lym_wilson_apply, projected form) in
+    complex128 and complex64, timed with HIP events on the stream the
+    kernels are launched on; GB/s from the compulsory bytes per site."""
+    import ctypes
+    import torch
+    from tfleonardym_amd import _lib
+    desc = op.ensure_f32()
+    nc, V = op.number_of_colors, op.volume
+    rows = nc - 1 if desc.link_format == _lib.LINKS_SU else nc
+    stream = torch.cuda.current_stream()
+    res = {}
+    for prec, name, w in ((_lib.PREC_F64, "f64", 8), (_lib.PREC_F32, "f32", 4)):
+        dt = torch.complex128 if prec == _lib.PREC_F64 else torch.complex64
+        x = src.to(dt)
+        y = torch.empty_like(x)
+
+        def launch():
+            _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), prec, 0, _lib.ptr(x), _lib.ptr(y),
+                                                   _lib.stream_handle()))
+        for _ in range(5):
+            launch()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            launch()
+        e1.record(stream)
+        e1.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        bps = bytes_per_site_dslash(nc, 4, w, w, rows)
+        gbs = bps * V / (us * 1e-6) / 1e9
+        res[name] = dict(us=us, bytes_per_site=bps, gbs=gbs, frac=gbs / HBM_PEAK_GBS)
+    return res
+
+
+def dslash_c3(reps, geometry=None):
+    """D at 24^3 x 48 (seeded near-identity links, Gaussian input); another
+    4D geometry with the same generator if given."""
+    import torch
+    from tfleonardym_amd import dirac, fields, lattice
+    W = DSLASH_C3
+    g = tuple(geometry) if geometry else W["geometry"]
+    t0 = time.time()
+    U = fields.gauge_field(g, W["nc"], seed=W["seed"], kind="near_identity", eps=W["eps"])
+    src = torch.from_numpy(fields.gaussian_spinor(g, 4, W["nc"], seed=W["seed"] + 1)).cuda()
+    op = dirac.DiracWilsonOperator(lattice.Configuration(U, g, W["nc"], 4), W["kappa"], True,
+                                   representation="fundamental")
+    res = time_dslash(op, src, reps)
+    for name in ("f64", "f32"):
+        res[name]["traffic"] = measured_traffic(g, name)
+    res["geometry"] = list(g)
+    for name in ("f64", "f32"):
+        print(f"[bench] wilson {g} {name}: {res[name]['us']:.2f} us/launch, {res[name]['gbs']:.0f} GB/s "
+              f"({100 * res[name]['frac']:.1f}% of {HBM_PEAK_GBS:.0f})", file=sys.stderr, flush=True)
+    print(f"[bench] dslash set-up + timing {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    return res
+
+
+def measured_traffic(geometry, prec):
+    """HBM bytes per launch of D at `geometry` / `prec` from the committed
+    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE passes, calibrated on a copy
+    kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "wilson_tile_traffic.md")
+    if not os.path.exists(path):
+        return None
+    key = "x".join(str(L) for L in geometry)
+    import re
+    for line in open(path):
+        m = re.search(r"traffic_bytes_per_launch (\S+) (f32|f64) = (\d+)", line)
+        if m and m.group(1) == key and m.group(2) == prec:
+            return int(m.group(3))
+    return None
+
+
+def run_dslash(args):
+    """Only D (for kernel profiles) at --dslash-geometry, then a 256 MiB
+    device copy (the calibration kernel of the FETCH_SIZE / WRITE_SIZE
+    counters): one JSON line."""
+    import torch
+    geometry = tuple(int(v) for v in args.dslash_geometry.split("x"))
+    res = dslash_c3(args.dslash_reps, geometry)
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    c = torch.empty_like(a)
+    for _ in range(3):
+        c.copy_(a)
+    torch.cuda.synchronize()
+    f32 = res["f32"]
+    print(json.dumps({"metric": f"Wilson D GB/s ({args.dslash_geometry} SU(3), compulsory bytes)", "value": f32["gbs"],
+                      "unit": "GB/s", "n_gpus": 1, "steps": args.dslash_reps, "warmup": 5,
+                      "ms_per_step": f32["us"] * 1e-3, "higher_is_better": True, "scaling": "weak",
+                      "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded near-identity SU(3) links)",
+                      "config": {"workload": f"wilson_d_{args.dslash_geometry}_su3"}, "dslash": res}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(SERVING))
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(SERVING) + ["dslash"])
     ap.add_argument("--precision", default=None, choices=["mixed", "double", "half"],
                     help="Krylov precision (default: mixed; batched workloads: their own)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--dslash-reps", type=int, default=50)
+    ap.add_argument("--no-dslash-c3", action="store_true", help="skip the 24^3x48 dslash measurement")
+    ap.add_argument("--dslash-geometry", default="24x24x24x48", help="lattice of --workload dslash")
     ap.add_argument("--baseline-steps", type=int, default=2,
                     help="timed solves of the plain FP64 CG-M for the speed-up figure (0: skip)")
     args = ap.parse_args()
@@ -211,6 +313,8 @@ def main():
         sys.exit(2)
     if args.workload in BATCHED:
         return run_batched(args)
+    if args.workload == "dslash":
+        return run_dslash(args)
     if args.precision is None:
         args.precision = "mixed"
     if args.workload in SERVING:
@@ -277,29 +381,13 @@ def main():
               flush=True)
 
     # ---- Wilson kernel bandwidth (HIP events on the launch stream) ----------
-    desc = op.ensure_f32()
-    res = {}
-    stream = torch.cuda.current_stream()
-    for prec, name, wl in ((_lib.PREC_F64, "f64", 8), (_lib.PREC_F32, "f32", 4)):
-        dt = torch.complex128 if prec == _lib.PREC_F64 else torch.complex64
-        x = bd.to(dt)
-        y = torch.empty_like(x)
-        for _ in range(5):
-            _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), prec, 0, _lib.ptr(x), _lib.ptr(y),
-                                                   _lib.stream_handle()))
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(args.dslash_reps):
-            _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), prec, 0, _lib.ptr(x), _lib.ptr(y),
-                                                   _lib.stream_handle()))
-        e1.record(stream)
-        e1.synchronize()
-        us = e0.elapsed_time(e1) * 1e3 / args.dslash_reps
-        bps = bytes_per_site_dslash(nc, 4, wl, wl, nc - 1 if desc.link_format == _lib.LINKS_SU else nc)
-        res[name] = dict(us=us, bytes_per_site=bps, gbs=bps * V / (us * 1e-6) / 1e9)
-        print(f"[bench] wilson {name}: {us:.2f} us/launch, {res[name]['gbs']:.0f} GB/s", file=sys.stderr,
+    res = time_dslash(op, bd, args.dslash_reps)
+    for name in ("f64", "f32"):
+        print(f"[bench] wilson {name}: {res[name]['us']:.2f} us/launch, {res[name]['gbs']:.0f} GB/s", file=sys.stderr,
               flush=True)
+    res_c3 = None
+    if not args.no_dslash_c3:
+        res_c3 = dslash_c3(args.dslash_reps)
 
     # ---- CPU baseline: the oracle (NumPy restatement of the reference op
     # sequence), a bounded number of FP64 CG-M iterations, extrapolated ------
@@ -319,6 +407,7 @@ def main():
         print(f"[bench] cpu baseline {per_it:.2f} s/iteration", file=sys.stderr, flush=True)
 
     f32 = res["f32"]
+    desc = op.desc
     out = {
         "metric": "multishift-CG solves/s at 1e-10 true resid",
         "value": solves_per_s,
@@ -343,8 +432,10 @@ def main():
         "roofline": {"kernel": "Wilson D complex64, one RHS (k_wilson_tile)", "bound": "hbm",
                      "achieved": f32["gbs"], "link_format": "su-rebuilt" if desc.link_format == _lib.LINKS_SU
                      else "full",
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": f32["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": f32["gbs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(geometry, "f32"),
                      "bytes_per_site": f32["bytes_per_site"], "sites": V, "avg_us": f32["us"]},
+        "dslash_24c3x48": res_c3,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
