@@ -130,6 +130,7 @@ def run_batched(args):
     srcs = [torch.from_numpy(s).to(dev) for s in srcs]
     op = dirac.DiracWilsonOperator(lattice.Configuration(U, geometry, nc, 4), W["kappa"], True,
                                    representation="fundamental")
+    tiles = op.autotune() if not args.no_autotune else None
     A = dirac.SquareOperator(op)
     mode = args.precision or W["precision"]
     solver = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode=mode,
@@ -176,6 +177,7 @@ def run_batched(args):
         "iterations": solver.last_stats[0]["iterations"],
         "plain_fp64_single_rhs_solves_per_s": plain,
         "speedup_vs_plain_fp64": (value / plain) if plain else None,
+        "wilson_tiles": tiles,
     }
     print(json.dumps(out))
 
@@ -231,7 +233,7 @@ def time_dslash(op, src, reps):
     return res
 
 
-def dslash_c3(reps, geometry=None):
+def dslash_c3(reps, geometry=None, autotune=True):
     """D at 24^3 x 48 (seeded near-identity links, Gaussian input); another
     4D geometry with the same generator if given."""
     import torch
@@ -243,7 +245,9 @@ def dslash_c3(reps, geometry=None):
     src = torch.from_numpy(fields.gaussian_spinor(g, 4, W["nc"], seed=W["seed"] + 1)).cuda()
     op = dirac.DiracWilsonOperator(lattice.Configuration(U, g, W["nc"], 4), W["kappa"], True,
                                    representation="fundamental")
+    tiles = op.autotune() if autotune else {n: op.tile_info(n) for n in ("f64", "f32")}
     res = time_dslash(op, src, reps)
+    res["tiles"] = tiles
     for name in ("f64", "f32"):
         res[name]["traffic"] = measured_traffic(g, name)
     res["geometry"] = list(g)
@@ -276,7 +280,7 @@ def run_dslash(args):
     counters): one JSON line."""
     import torch
     geometry = tuple(int(v) for v in args.dslash_geometry.split("x"))
-    res = dslash_c3(args.dslash_reps, geometry)
+    res = dslash_c3(args.dslash_reps, geometry, autotune=not args.no_autotune)
     a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
     c = torch.empty_like(a)
     for _ in range(3):
@@ -303,6 +307,7 @@ def main():
     ap.add_argument("--dslash-reps", type=int, default=50)
     ap.add_argument("--no-dslash-c3", action="store_true", help="skip the 24^3x48 dslash measurement")
     ap.add_argument("--dslash-geometry", default="24x24x24x48", help="lattice of --workload dslash")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in Wilson tile rule")
     ap.add_argument("--baseline-steps", type=int, default=2,
                     help="timed solves of the plain FP64 CG-M for the speed-up figure (0: skip)")
     args = ap.parse_args()
@@ -336,6 +341,10 @@ def main():
 
     cfg = lattice.Configuration(U, geometry, nc, 4)
     op = dirac.DiracWilsonOperator(cfg, W["kappa"], True, representation="fundamental")
+    tiles = None
+    if not args.no_autotune:
+        tiles = op.autotune()
+        print(f"[bench] autotuned tiles: {tiles}", file=sys.stderr, flush=True)
     A = dirac.SquareOperator(op)
     bd = torch.from_numpy(b).to(dev)
     solver = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10,
@@ -387,7 +396,7 @@ def main():
               flush=True)
     res_c3 = None
     if not args.no_dslash_c3:
-        res_c3 = dslash_c3(args.dslash_reps)
+        res_c3 = dslash_c3(args.dslash_reps, autotune=not args.no_autotune)
 
     # ---- CPU baseline: the oracle (NumPy restatement of the reference op
     # sequence), a bounded number of FP64 CG-M iterations, extrapolated ------
@@ -436,6 +445,8 @@ def main():
                      "traffic": measured_traffic(geometry, "f32"),
                      "bytes_per_site": f32["bytes_per_site"], "sites": V, "avg_us": f32["us"]},
         "dslash_24c3x48": res_c3,
+        "wilson_tiles": {"16c3x32": tiles or {"f32": op.tile_info("f32"), "f64": op.tile_info("f64")},
+                         "24c3x48": res_c3.get("tiles") if res_c3 else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -132,6 +132,26 @@ int lym_convert(int prec_in, const void* in, int prec_out, void* out, int64_t n,
  * conditions materialised by dirac.py:351-378 leave links that fail it. */
 int lym_link_deviation(const lym_wilson* op, int prec, double* dev_out, void* stream);
 
+/* ---- kernel autotuning (no reference counterpart) ----------------------
+ * The LDS-tiled Wilson kernel (4D fundamental lattices with L0 divisible
+ * by 8) works on tiles of a0 x a1 x k x L3 sites with `hif` hops' loads in
+ * flight per wave.  lym_tune_wilson times D (projected form) for every
+ * admissible tile and hop grouping, and the untiled kernel, on the caller's
+ * fields `in` / `out` (op's layout, precision `prec`), `reps` launches
+ * each, and records the fastest for this lattice, Nc and precision in a
+ * process-wide table that every later call on such an operator uses.
+ * Synchronous.  lym_tile_info reports the choice in effect (tuned, forced
+ * by LYM_TILE / LYM_HIF, or the built-in rule); a0 = 0 means untiled. */
+typedef struct lym_tile_choice {
+  int32_t a0, a1, k, hif;
+  int32_t nsite;        /* sites (threads) per block */
+  int32_t tuned;        /* 1: from lym_tune_wilson */
+  double us;            /* measured time per launch (lym_tune_wilson only) */
+} lym_tile_choice;
+int lym_tune_wilson(const lym_wilson* op, int prec, int reps, const void* in, void* out, lym_tile_choice* best,
+                    void* stream);
+int lym_tile_info(const lym_wilson* op, int prec, lym_tile_choice* out);
+
 /* ---- solvers ---------------------------------------------------------- */
 /* Device workspace (bytes) for the solvers below. */
 size_t lym_solver_workspace_bytes(const lym_wilson* op, int prec, int nshift);

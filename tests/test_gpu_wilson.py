@@ -127,6 +127,36 @@ def test_lds_tile_shapes(prec, geometry, monkeypatch):
         monkeypatch.delenv("LYM_HIF")
 
 
+def test_autotune_records_and_keeps_parity():
+    """lym_tune_wilson times every tile / hop grouping and the untiled kernel,
+    records the fastest for this lattice; later applications use it (tile
+    info says tuned) and still give the oracle's D, in both precisions.
+    (A lattice no other test uses: the tuned table lives for the process.)"""
+    import ctypes
+    from tfleonardym_amd import _lib
+    geometry = (16, 4, 4, 16)
+    U = fields.gauge_field(geometry, 3, seed=91, kind="haar")
+    psi = fields.gaussian_spinor(geometry, 4, 3, seed=92)
+    op = make_op(U, geometry, 0.121, True, "fundamental", -1.0)
+    want = ow.WilsonOracle(U, geometry, 0.121, True, "fundamental", bc_factor=-1.0).apply(psi)
+    assert op.tile_info("f32")["tuned"] == 0
+    choices = op.autotune(reps=3)
+    for name, tol in (("f64", TOL64), ("f32", TOL32)):
+        c = choices[name]
+        assert c["tuned"] == 1 and c["us"] > 0
+        info = op.tile_info(name)
+        assert info["tuned"] == 1 and (info["a0"], info["a1"], info["k"]) == (c["a0"], c["a1"], c["k"])
+        if c["a0"]:
+            assert geometry[1] % c["a1"] == 0 and geometry[2] % c["k"] == 0 and c["nsite"] % 64 == 0
+        dt = torch.complex128 if name == "f64" else torch.complex64
+        x = torch.from_numpy(psi).to("cuda").to(dt)
+        y = torch.empty_like(x)
+        prec = _lib.PREC_F64 if name == "f64" else _lib.PREC_F32
+        _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(op.desc), prec, 0, _lib.ptr(x), _lib.ptr(y),
+                                               _lib.stream_handle()))
+        assert rel(y.to(torch.complex128), want) < tol, (name, c)
+
+
 @pytest.mark.parametrize("nc", [2, 3])
 def test_su_link_rebuild(nc, monkeypatch):
     """SU(N) links are flagged LINKS_SU and the kernels rebuild each link's

@@ -42,6 +42,22 @@ class LymWilson(ctypes.Structure):
     ]
 
 
+class LymTileChoice(ctypes.Structure):
+    """lym_tile_choice: LDS tile (a0, a1, k; a0 = 0: untiled) and hops in flight."""
+    _fields_ = [
+        ("a0", ctypes.c_int32),
+        ("a1", ctypes.c_int32),
+        ("k", ctypes.c_int32),
+        ("hif", ctypes.c_int32),
+        ("nsite", ctypes.c_int32),
+        ("tuned", ctypes.c_int32),
+        ("us", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class LymSolveStats(ctypes.Structure):
     _fields_ = [
         ("status", ctypes.c_int32),
@@ -86,6 +102,8 @@ SIGNATURES = {
     "lym_gamma5": (_i32, [_i32, _i32, _i64, _vp, _vp, _vp]),
     "lym_convert": (_i32, [_i32, _vp, _i32, _vp, _i64, _vp]),
     "lym_link_deviation": (_i32, [_vp, _i32, _vp, _vp]),
+    "lym_tune_wilson": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _vp]),
+    "lym_tile_info": (_i32, [_vp, _i32, _vp]),
     "lym_solver_workspace_bytes": (_sz, [ctypes.POINTER(LymWilson), _i32, _i32]),
     "lym_mr_multishift": (_i32, [ctypes.POINTER(LymWilson), _i32, ctypes.POINTER(_f64), _i32, _vp,
                                  ctypes.POINTER(_vp), _i32, _f64, _f64, _vp, _sz,

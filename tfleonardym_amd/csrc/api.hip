@@ -1,11 +1,16 @@
 // extern "C" entry points of liblym.so (declared in include/lym.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
+#include <vector>
 
 #include "lym_internal.h"
 
@@ -21,6 +26,47 @@ int hip_fail(hipError_t e, const char* what) {
 }
 
 int64_t field_elems(const OpDesc& d) { return (int64_t)d.ncomp * d.lat.V; }
+
+// Every LDS tile a 4D fundamental lattice admits (L0 divisible by the XCD
+// count): a0 | L0/8, a1 | L1, k | L2, a0*a1*k*L3 a multiple of 64 and at
+// most max_threads sites, lds_per_site * sites within the LDS budget.
+static std::vector<TileMap> admissible_tiles(const lym_wilson* w, int adj, int64_t lds_per_site, int max_threads) {
+  std::vector<TileMap> out;
+  if (w->ndim != 4 || adj || w->dims[0] % kNumXcd) return out;
+  const int L0 = w->dims[0], L1 = w->dims[1], L2 = w->dims[2], T = w->dims[3];
+  const int64_t V = (int64_t)L0 * L1 * L2 * T;
+  const int slab = L0 / kNumXcd;
+  for (int a0 = 1; a0 <= slab; ++a0) {
+    if (slab % a0) continue;
+    for (int a1 = 1; a1 <= L1; ++a1) {
+      if (L1 % a1) continue;
+      for (int k = 1; k <= L2; ++k) {
+        if (L2 % k) continue;
+        const int64_t ns = (int64_t)a0 * a1 * k * T;
+        if (ns % 64 || ns > max_threads || ns * lds_per_site > kTileMaxLds) continue;
+        out.push_back(TileMap{1, a0, a1, k, T, slab, slab / a0, L1 / a1, L2 / k, (int)ns, (int)(V / ns), 1});
+      }
+    }
+  }
+  return out;
+}
+
+// Tiles chosen by lym_tune_wilson, per (lattice, Nc, representation,
+// precision index), for the whole process.
+struct TuneKey {
+  int ndim, nc, adj, pi, L[4];
+  bool operator<(const TuneKey& o) const {
+    return std::tie(ndim, nc, adj, pi, L[0], L[1], L[2], L[3]) <
+           std::tie(o.ndim, o.nc, o.adj, o.pi, o.L[0], o.L[1], o.L[2], o.L[3]);
+  }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TileMap> g_tuned;
+static TuneKey tune_key(const lym_wilson* w, int adj, int pi) {
+  TuneKey k{w->ndim, w->nc, adj, pi, {1, 1, 1, 1}};
+  for (int mu = 0; mu < w->ndim && mu < 4; ++mu) k.L[mu] = w->dims[mu];
+  return k;
+}
 
 int make_desc(const lym_wilson* w, OpDesc* d) {
   if (!w) {
@@ -122,43 +168,40 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   // lds_per_site: LDS bytes a tile site needs; [lo, hi]: preferred tile sizes
   auto choose_tile = [&](int64_t lds_per_site, int max_threads, int lo, int hi) {
     TileMap best{};
-    if (w->ndim != 4 || o.adj || w->dims[0] % kNumXcd) return best;
-    const int L0 = w->dims[0], L1 = w->dims[1], L2 = w->dims[2], T = w->dims[3];
-    const int slab = L0 / kNumXcd;
+    const std::vector<TileMap> all = admissible_tiles(w, o.adj, lds_per_site, max_threads);
+    const int L0 = w->dims[0], L1 = w->dims[1], L2 = w->dims[2];
     auto part = [](int a, int L) { return a == L ? 0.0 : 2.0 / a; };
     for (int pass = 0; pass < 2 && !best.on; ++pass) {
       double best_cost = 1e30;
-      for (int a0 = 1; a0 <= slab; ++a0) {
-        if (slab % a0) continue;
-        for (int a1 = 1; a1 <= L1; ++a1) {
-          if (L1 % a1) continue;
-          for (int k = 1; k <= L2; ++k) {
-            if (L2 % k) continue;
-            const int64_t ns = (int64_t)a0 * a1 * k * T;
-            if (ns % 64 || ns > max_threads) continue;
-            if (ns * lds_per_site > kTileMaxLds) continue;
-            const bool forced = want[0] && a0 == want[0] && a1 == want[1] && k == want[2];
-            if (pass == 0 && !forced && (ns < lo || ns > hi)) continue;
-            const double cost = forced ? -1.0 : 1.0 + part(a0, L0) + part(a1, L1) + part(k, L2);
-            const bool tie = std::fabs(cost - best_cost) < 1e-9;
-            if (cost < best_cost - 1e-9 || (tie && (k > best.k || (k == best.k && ns < best.nsite)))) {
-              best_cost = cost;
-              best = TileMap{1, a0, a1, k, T, slab, slab / a0, L1 / a1, L2 / k, (int)ns, (int)(V / ns)};
-            }
-          }
+      for (const TileMap& t : all) {
+        const bool forced = want[0] && t.a0 == want[0] && t.a1 == want[1] && t.k == want[2];
+        if (pass == 0 && !forced && (t.nsite < lo || t.nsite > hi)) continue;
+        const double cost = forced ? -1.0 : 1.0 + part(t.a0, L0) + part(t.a1, L1) + part(t.k, L2);
+        const bool tie = std::fabs(cost - best_cost) < 1e-9;
+        if (cost < best_cost - 1e-9 || (tie && (t.k > best.k || (t.k == best.k && t.nsite < best.nsite)))) {
+          best_cost = cost;
+          best = t;
         }
       }
     }
-    best.hif = 1;
     return best;
   };
+  const char* hf = std::getenv("LYM_HIF");
+  const int want_hif = hf ? std::atoi(hf) : 0;
   for (int pi = 0; pi < 2; ++pi) {
     const int64_t es = pi == 0 ? 16 : 8;
     TileMap best = choose_tile(o.ncomp * es, pi == 0 ? kTileMaxThreads64 : kTileMaxThreads, 256, 512);
-    if (const char* hf = std::getenv("LYM_HIF")) {
-      const int h = std::atoi(hf);
-      if (h == 2 || h == 4) best.hif = h;
+    o.tile_tuned[pi] = 0;
+    if (!want[0] && !want_hif && o.use_tile) {
+      // a shape recorded by lym_tune_wilson for this lattice wins
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      auto it = g_tuned.find(tune_key(w, o.adj, pi));
+      if (it != g_tuned.end()) {
+        best = it->second;
+        o.tile_tuned[pi] = 1;
+      }
     }
+    if (want_hif == 2 || want_hif == 4) best.hif = want_hif;
     o.tile[pi] = best;
   }
   // multi-RHS tile (complex64): kMrRhs spinors per site in LDS.  Opt-in
@@ -283,6 +326,98 @@ int lym_axpby(int prec, int64_t n, double a_re, double a_im, const void* x, doub
   }
   if (n == 0) return LYM_OK;
   return blas_axpby(prec, n, a_re, a_im, x, b_re, b_im, y, (hipStream_t)stream);
+}
+
+static void fill_choice(const TileMap& t, int tuned, double us, lym_tile_choice* c) {
+  c->a0 = t.on ? t.a0 : 0;
+  c->a1 = t.on ? t.a1 : 0;
+  c->k = t.on ? t.k : 0;
+  c->hif = t.on ? (t.hif ? t.hif : 1) : 0;
+  c->nsite = t.on ? t.nsite : 0;
+  c->tuned = tuned;
+  c->us = us;
+}
+
+int lym_tile_info(const lym_wilson* op, int prec, lym_tile_choice* out) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (!out || (prec != LYM_PREC_F64 && prec != LYM_PREC_F32 && prec != LYM_PREC_F16)) {
+    set_error("lym_tile_info: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  const int pi = prec == LYM_PREC_F64 ? 0 : 1;
+  fill_choice(wilson_tiled(d, prec) ? d.tile[pi] : TileMap{}, d.tile_tuned[pi], 0.0, out);
+  return LYM_OK;
+}
+
+int lym_tune_wilson(const lym_wilson* op, int prec, int reps, const void* in, void* out, lym_tile_choice* best,
+                    void* stream) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (!in || !out || in == out || reps < 1 || (prec != LYM_PREC_F64 && prec != LYM_PREC_F32)) {
+    set_error("lym_tune_wilson: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int pi = prec == LYM_PREC_F64 ? 0 : 1;
+  // candidates: the untiled kernel, then every tile x hops in flight
+  std::vector<TileMap> cands{TileMap{}};
+  for (const TileMap& t : admissible_tiles(op, d.adj, (int64_t)d.ncomp * (pi == 0 ? 16 : 8),
+                                           pi == 0 ? kTileMaxThreads64 : kTileMaxThreads)) {
+    for (int hif : {1, 2, 4}) {
+      if (hif == 4 && prec != LYM_PREC_F32) continue;
+      if (hif > 1 && t.nsite > 512) continue;  // the launcher runs those with one hop in flight
+      TileMap c = t;
+      c.hif = hif;
+      cands.push_back(c);
+    }
+  }
+  hipEvent_t e0, e1;
+  LYM_HIP(hipEventCreate(&e0));
+  LYM_HIP(hipEventCreate(&e1));
+  EpiArgs a;
+  a.out.data = out;
+  Field f;
+  f.data = const_cast<void*>(in);
+  double best_us = 1e30;
+  TileMap best_t{};
+  // each candidate: the best of 3 timed runs of `reps` launches (run-to-run
+  // noise is a few percent, the spread between good shapes not much more);
+  // the first candidate is warmed up twice as long
+  for (size_t ci = 0; ci < cands.size() && !rc; ++ci) {
+    const TileMap& c = cands[ci];
+    OpDesc dc = d;
+    dc.use_tile = c.on ? 1 : 0;
+    dc.use_mr = 0;
+    dc.tile[pi] = c;
+    const int warm = ci == 0 ? 2 * reps : 2;
+    for (int i = 0; i < warm && !rc; ++i) rc = wilson_launch(EPI_STORE, dc, prec, true, f, a, RedCtl{}, st);
+    double us = 1e30;
+    for (int trial = 0; trial < 3 && !rc; ++trial) {
+      if (hipEventRecord(e0, st) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventRecord");
+      for (int i = 0; i < reps && !rc; ++i) rc = wilson_launch(EPI_STORE, dc, prec, true, f, a, RedCtl{}, st);
+      if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventRecord");
+      float ms = 0.f;
+      if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventSynchronize");
+      if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventElapsedTime");
+      if (!rc) us = std::min(us, 1e3 * ms / reps);
+    }
+    if (!rc && us < best_us) {
+      best_us = us;
+      best_t = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned[tune_key(op, d.adj, pi)] = best_t;
+  }
+  if (best) fill_choice(best_t, 1, best_us, best);
+  return LYM_OK;
 }
 
 int lym_link_deviation(const lym_wilson* op, int prec, double* dev_out, void* stream) {

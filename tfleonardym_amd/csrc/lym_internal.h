@@ -27,6 +27,7 @@ struct OpDesc {
   int wblocks = 0;   // plain Wilson grid size (blocks)
   int wthreads = 0;  // plain Wilson block size (threads, multiple of 64)
   TileMap tile[2]{};  // LDS-tiled kernel per precision: [0] complex128, [1] complex64
+  int tile_tuned[2]{};  // the tile came from lym_tune_wilson
   int use_tile = 1;   // 0 forces the plain kernel
   TileMap tile_mr{};  // multi-RHS LDS tile (complex64, kMrRhs spinors per site)
   int use_mr = 0;     // 1: batched complex64 launches use the multi-RHS kernel

@@ -274,6 +274,34 @@ class DiracWilsonOperator(Operator):
             d.link_format = _lib.su_link_format(d, links.device)
         return d
 
+    def autotune(self, reps=20, precisions=("f64", "f32")):
+        """Time every LDS tile shape and hop grouping of the Wilson kernel (and
+        the untiled kernel) for this operator's lattice and keep the fastest
+        for the rest of the process (lym_tune_wilson).  Returns the choices.
+        Later solves use the tuned kernel; results stay within rounding of
+        the untuned ones (block sums of the reductions regroup)."""
+        desc = self.ensure_f32()
+        shape = self.spinor_shape
+        g = torch.Generator(device="cpu").manual_seed(12345)
+        out = {}
+        for name in precisions:
+            prec, dt = (_lib.PREC_F64, torch.complex128) if name == "f64" else (_lib.PREC_F32, torch.complex64)
+            x = torch.randn(shape, dtype=dt, generator=g).to(self._links.device)
+            y = torch.empty_like(x)
+            choice = _lib.LymTileChoice()
+            _lib.check(_lib.lib().lym_tune_wilson(ctypes.byref(desc), prec, int(reps), _lib.ptr(x), _lib.ptr(y),
+                                                  ctypes.byref(choice), _lib.stream_handle(self._links.device)),
+                       "autotune")
+            out[name] = choice.as_dict()
+        return out
+
+    def tile_info(self, precision="f32"):
+        """The Wilson-kernel tile in effect for this operator (a0 = 0: untiled)."""
+        prec = {"f64": _lib.PREC_F64, "f32": _lib.PREC_F32, "f16": _lib.PREC_F16}[precision]
+        choice = _lib.LymTileChoice()
+        _lib.check(_lib.lib().lym_tile_info(ctypes.byref(self._desc), prec, ctypes.byref(choice)), "tile info")
+        return choice.as_dict()
+
     def ensure_f32(self):
         """complex64 copy of the links for the mixed-precision solvers."""
         if self._links32 is None:
