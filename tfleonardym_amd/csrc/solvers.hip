@@ -10,7 +10,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -697,16 +699,122 @@ static int ensure_sync() {
   return LYM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// chunk graphs: a chunk is a fixed launch sequence whose behaviour is driven
+// by device state, so it is captured once into a HIP graph and replayed with
+// one hipGraphLaunch per chunk instead of tens of kernel launches.  The key
+// holds everything the launches bake in (operator description, workspace,
+// solver kind, precision, batch).  Per host thread: at most kGraphCache
+// executables; the least recently used one not launched by the current solve
+// is dropped (a solve ends with a stream synchronisation, so earlier solves'
+// graphs have finished).  LYM_NO_GRAPH=1 in the environment launches the
+// kernels directly.
+// ---------------------------------------------------------------------------
+constexpr int kGraphCache = 8;
+
+struct KeyBuf {
+  std::string s;
+  template <class T>
+  KeyBuf& add(const T& v) {
+    s.append(reinterpret_cast<const char*>(&v), sizeof(T));
+    return *this;
+  }
+};
+
+static std::string op_key(const OpDesc& d) {
+  KeyBuf k;
+  k.add(d.ndim).add(d.nc).add(d.adj).add(d.herm).add(d.ncomp);
+  for (int mu = 0; mu < 4; ++mu) k.add(d.lat.L[mu]);
+  k.add(d.lat.V).add(d.lat.kappa).add(d.lat.bc_re).add(d.lat.bc_im).add(d.lat.has_bc);
+  k.add(d.map.tiled).add(d.map.k).add(d.wblocks).add(d.wthreads).add(d.use_tile).add(d.use_mr).add(d.link_su);
+  for (const TileMap* t : {&d.tile[0], &d.tile[1], &d.tile_mr})
+    k.add(t->on).add(t->a0).add(t->a1).add(t->k).add(t->hif);
+  k.add(d.links64).add(d.linksr64).add(d.links32).add(d.linksr32);
+  return k.s;
+}
+
+struct GraphEntry {
+  std::string key;
+  hipGraphExec_t exec = nullptr;
+  uint64_t used = 0;   // tick of the last launch
+  uint64_t solve = 0;  // solve that last launched it
+};
+struct GraphCache {
+  std::vector<GraphEntry> e;
+  hipStream_t cap = nullptr;
+  int device = -1;
+  uint64_t tick = 0, solve = 0;
+};
+static thread_local GraphCache g_graphs;
+
+static bool graphs_enabled() {
+  const char* v = std::getenv("LYM_NO_GRAPH");
+  return !(v && v[0] == '1');
+}
+
+// a new solve: graphs launched before it have completed
+static void graph_new_solve() { ++g_graphs.solve; }
+
+// Launch `chunk` on `st` through its cached graph (captured on first use).
+template <class F>
+static int launch_chunk(const std::string& key, hipStream_t st, F&& chunk) {
+  if (!graphs_enabled()) return chunk(st);
+  GraphCache& G = g_graphs;
+  int dev = 0;
+  LYM_HIP(hipGetDevice(&dev));
+  if (G.device != dev) {  // executables and the capture stream belong to one device
+    for (auto& x : G.e) (void)hipGraphExecDestroy(x.exec);
+    G.e.clear();
+    if (G.cap) (void)hipStreamDestroy(G.cap);
+    G.cap = nullptr;
+    G.device = dev;
+  }
+  GraphEntry* hit = nullptr;
+  for (auto& x : G.e)
+    if (x.key == key) hit = &x;
+  if (!hit) {
+    if (!G.cap) LYM_HIP(hipStreamCreateWithFlags(&G.cap, hipStreamNonBlocking));
+    LYM_HIP(hipStreamBeginCapture(G.cap, hipStreamCaptureModeThreadLocal));
+    const int rc = chunk(G.cap);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(G.cap, &g);
+    if (rc || ec != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : hip_fail(ec, "hipStreamEndCapture");
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return hip_fail(ei, "hipGraphInstantiate");
+    if ((int)G.e.size() >= kGraphCache) {
+      int victim = -1;
+      for (int i = 0; i < (int)G.e.size(); ++i)
+        if (G.e[i].solve != G.solve && (victim < 0 || G.e[i].used < G.e[victim].used)) victim = i;
+      if (victim >= 0) {
+        (void)hipGraphExecDestroy(G.e[victim].exec);
+        G.e.erase(G.e.begin() + victim);
+      }
+    }
+    G.e.push_back(GraphEntry{key, ex, 0, 0});
+    hit = &G.e.back();
+  }
+  hit->used = ++G.tick;
+  hit->solve = G.solve;
+  LYM_HIP(hipGraphLaunch(hit->exec, st));
+  return LYM_OK;
+}
+
 // Enqueue `chunk` (a fixed launch sequence), keeping two chunks in flight,
 // until the device reports PH_DONE.  `on_poll(status4)` may enqueue extra
 // work between chunks (e.g. a precision switch); `max_chunks` guards the loop.
 template <class F, class G>
-static int run_chunks(SolverState* st_dev, int64_t max_chunks, hipStream_t stream, F&& chunk, G&& on_poll) {
+static int run_chunks(SolverState* st_dev, int64_t max_chunks, hipStream_t stream, const std::string& key, F&& chunk,
+                      G&& on_poll) {
   int rc = ensure_sync();
   if (rc) return rc;
   int64_t issued = 0, polled = 0;
   auto issue = [&](int slot) -> int {
-    const int e = chunk();
+    const int e = launch_chunk(key, stream, chunk);
     if (e) return e;
     LYM_HIP(hipMemcpyAsync(&g_sync.pinned[4 * slot], st_dev, 4 * sizeof(int), hipMemcpyDeviceToHost, stream));
     LYM_HIP(hipEventRecord(g_sync.ev[slot], stream));
@@ -784,6 +892,7 @@ static void init_cgm_state(SolverState& h, int nshift, const double* shifts, con
 static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Field* xs, Field r, Field t,
                     int nshift, const double* shifts, const double* tol_rel, int max_iter, hipStream_t st,
                     SolverState* out_state) {
+  graph_new_solve();
   SolverState h;
   init_cgm_state(h, nshift, shifts, tol_rel, max_iter);
   h.phase = PH_K1;
@@ -810,7 +919,7 @@ static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Fi
   if (rc) return rc;
   const Field pbase = P.p[h.base];
   const int per_chunk = 8;
-  auto chunk = [&]() -> int {
+  auto chunk = [&](hipStream_t st) -> int {
     for (int j = 0; j < per_chunk; ++j) {
       EpiArgs a1;
       a1.out = t;
@@ -836,7 +945,10 @@ static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Fi
     }
     return LYM_OK;
   };
-  rc = run_chunks(P.st, (int64_t)max_iter / per_chunk + 8, st, chunk, [](const int*) { return (int)LYM_OK; });
+  KeyBuf key;
+  key.add('C').add(prec).add(nshift).add(P.st).add(P.xs_dev).add(r.data).add(t.data).add(pbase.data);
+  rc = run_chunks(P.st, (int64_t)max_iter / per_chunk + 8, st, op_key(d) + key.s, chunk,
+                  [](const int*) { return (int)LYM_OK; });
   if (rc) return rc;
   LYM_HIP(hipMemcpyAsync(out_state, P.st, sizeof(SolverState), hipMemcpyDeviceToHost, st));
   LYM_HIP(hipStreamSynchronize(st));
@@ -853,6 +965,7 @@ static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Fi
 static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void* const* xs, int nrhs, int nshift,
                     const double* shifts, const double* tol_rel, int max_iter, double switch_tol, int lowprec,
                     hipStream_t st, std::vector<SolverState>& out_states) {
+  graph_new_solve();
   SolverState h;
   init_cgm_state(h, nshift, shifts, tol_rel, max_iter);
   h.phase = PH_KP;
@@ -885,7 +998,7 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   if (rc) return rc;
 
   int prec = P64;
-  auto chunk = [&]() -> int {
+  auto chunk = [&](hipStream_t st) -> int {
     const Field ring = prec == P64 ? P.ring64 : P.ring32;
     const Field pl = prec == P64 ? P.plive64 : P.plive32;
     const Field t = prec == P64 ? P.tb64 : P.tb32;
@@ -934,8 +1047,11 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   if (rce) return rce;
   const int64_t max_chunks = (int64_t)max_iter / kFastBlock64 + 8;
   int64_t issued = 0, polled = 0;
+  KeyBuf kb;
+  kb.add('F').add(nrhs).add(nshift).add(P.st).add(P.xs_dev).add(P.ring64.data);
+  const std::string key = op_key(d) + kb.s;
   auto issue = [&](int slot) -> int {
-    const int e = chunk();
+    const int e = launch_chunk(key + (prec == P64 ? 'd' : 's'), st, chunk);
     if (e) return e;
     LYM_HIP(hipMemcpyAsync(&g_sync.pinned[4 * slot], P.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
     LYM_HIP(hipEventRecord(g_sync.ev[slot], st));
@@ -987,6 +1103,7 @@ constexpr int kRelChunk = 8;  // iterations per host chunk
 
 static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* const* bs, void* const* xs, int nrhs,
                        double sigma, double tol, int max_iter, hipStream_t st, std::vector<SolverState>& out_states) {
+  graph_new_solve();
   // site kernels instantiated per storage precision of r, p and component count
   auto dispatch = [&](auto f) -> int {
     auto by_nc = [&](auto pv) -> int {
@@ -1039,7 +1156,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   // the update accumulation is mostly gated off: a small grid-stride grid
   const int gsr = std::min(gs, 1024);
   static_assert(kRelChunk % 2 == 0, "the last iteration of a chunk must carry the update kernels");
-  auto chunk = [&]() -> int {
+  auto chunk = [&](hipStream_t st) -> int {
     for (int j = 0; j < kRelChunk; ++j) {
       // x_lo += alpha p (last step), p = r + beta p
       int e = dispatch([&](auto pv, auto nc) {
@@ -1095,8 +1212,11 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   // an update takes an iteration slot without advancing iter: at most one per iteration
   const int64_t max_chunks = 2 * ((int64_t)max_iter + kRelChunk) / kRelChunk + 8;
   int64_t issued = 0, polled = 0;
+  KeyBuf kb;
+  kb.add('R').add(pl).add(nrhs).add(P.st).add(P.xs_dev).add(P.rl_r.data);
+  const std::string key = op_key(d) + kb.s;
   auto issue = [&](int slot) -> int {
-    const int e = chunk();
+    const int e = launch_chunk(key, st, chunk);
     if (e) return e;
     LYM_HIP(hipMemcpyAsync(&g_sync.pinned[4 * slot], P.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
     LYM_HIP(hipEventRecord(g_sync.ev[slot], st));
@@ -1292,6 +1412,7 @@ int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const
              int max_steps, double precision, double omega, void* ws, size_t ws_bytes, lym_solve_stats* stats,
              hipStream_t st) {
   const auto t0 = std::chrono::steady_clock::now();
+  graph_new_solve();
   if (nshift < 1 || nshift > kMaxShifts || first < 0 || first >= nshift) {
     set_error("bad shift list for the multishift MR solver");
     return LYM_ERR_ARG;
@@ -1329,7 +1450,7 @@ int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const
                      first, nshift);
   LYM_HIP(hipGetLastError());
   const int per_chunk = 8;
-  auto chunk = [&]() -> int {
+  auto chunk = [&](hipStream_t st) -> int {
     for (int j = 0; j < per_chunk; ++j) {
       EpiArgs a1;
       a1.out = P.t64;
@@ -1350,7 +1471,10 @@ int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const
     }
     return LYM_OK;
   };
-  int rc = run_chunks(P.st, (int64_t)max_steps / per_chunk + 8, st, chunk, [](const int*) { return (int)LYM_OK; });
+  KeyBuf key;
+  key.add('M').add(nshift).add(P.st).add(P.xs_dev);
+  int rc = run_chunks(P.st, (int64_t)max_steps / per_chunk + 8, st, op_key(d) + key.s, chunk,
+                      [](const int*) { return (int)LYM_OK; });
   if (rc) return rc;
   LYM_HIP(hipMemcpyAsync(&h, P.st, sizeof(SolverState), hipMemcpyDeviceToHost, st));
   LYM_HIP(hipStreamSynchronize(st));
