@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define LYM_ABI_VERSION 2
+#define LYM_ABI_VERSION 3
 
 enum {
   LYM_OK = 0,
@@ -131,6 +131,22 @@ int lym_convert(int prec_in, const void* in, int prec_out, void* out, int64_t n,
  * counterpart (the reference always reads the full matrix); the boundary
  * conditions materialised by dirac.py:351-378 leave links that fail it. */
 int lym_link_deviation(const lym_wilson* op, int prec, double* dev_out, void* stream);
+
+/* ---- fermion force ------------------------------------------------------
+ * out (+)= sum_p weights[p] * [ L(xs[p], ys[p]) + symmetric * L(ys[p], xs[p]) ]
+ * with L(X, Y) = DiracWilsonOperator.lie_derivative(X, Y) (dirac.py:525-565):
+ * a (D, nc, nc, V) complex128 field kappa * sum_k T^k l_k(X, Y) on the
+ * operator's links (boundary factor included).  symmetric = 1 with
+ * ys[p] = Op xs[p] and weights = the rational alphas is the sum
+ * n_flavor.force builds (fermion_action.py:64-93).  Spinors are complex128 in
+ * the operator's layout: (Ns, nc, V), or (Ns, nc, nc, V) matrices for the
+ * adjoint representation (the reference's conversion to x^a = 2 tr(T^a X)
+ * and its real adjoint links are folded in; the result is identical).
+ * accumulate = 0 overwrites out (npairs = 0 then zeroes it), 1 adds to it.
+ * xs, ys, weights: host arrays of npairs entries (device pointers in xs/ys);
+ * no entry may alias out.  F64 only.  Asynchronous on `stream`. */
+int lym_lie_derivative(const lym_wilson* op, int npairs, const void* const* xs, const void* const* ys,
+                       const double* weights, int symmetric, int accumulate, void* out, void* stream);
 
 /* ---- kernel autotuning (no reference counterpart) ----------------------
  * The LDS-tiled Wilson kernel (4D fundamental lattices with L0 divisible

@@ -216,6 +216,7 @@ class WilsonOracle:
         self.geometry = tuple(int(L) for L in geometry)
         self.tables = olat.lookup_tables(self.geometry)
         self.links_orig = np.asarray(links_orig, dtype=np.complex128)
+        self.bc_factor = bc_factor
         if bc_factor is None or bc_factor == 1:
             self.links = self.links_orig
         else:

@@ -15,7 +15,7 @@ PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 REP_FUNDAMENTAL, REP_ADJOINT = 0, 1
 MODE_PROJECTED, MODE_SIMPLE = 0, 1
 LINKS_FULL, LINKS_SU = 0, 1
-ABI_VERSION = 2  # LYM_ABI_VERSION of include/lym.h
+ABI_VERSION = 3  # LYM_ABI_VERSION of include/lym.h
 
 OK = 0
 NOT_CONVERGED = 1
@@ -102,6 +102,8 @@ SIGNATURES = {
     "lym_gamma5": (_i32, [_i32, _i32, _i64, _vp, _vp, _vp]),
     "lym_convert": (_i32, [_i32, _vp, _i32, _vp, _i64, _vp]),
     "lym_link_deviation": (_i32, [_vp, _i32, _vp, _vp]),
+    "lym_lie_derivative": (_i32, [ctypes.POINTER(LymWilson), _i32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                  ctypes.POINTER(_f64), _i32, _i32, _vp, _vp]),
     "lym_tune_wilson": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _vp]),
     "lym_tile_info": (_i32, [_vp, _i32, _vp]),
     "lym_solver_workspace_bytes": (_sz, [ctypes.POINTER(LymWilson), _i32, _i32]),

@@ -12,6 +12,7 @@
 #include <tuple>
 #include <vector>
 
+#include "force_site.h"
 #include "lym_internal.h"
 
 namespace lym {
@@ -429,6 +430,41 @@ int lym_link_deviation(const lym_wilson* op, int prec, double* dev_out, void* st
     return LYM_ERR_ARG;
   }
   return link_deviation(d, prec, dev_out, (hipStream_t)stream);
+}
+
+int lym_lie_derivative(const lym_wilson* op, int npairs, const void* const* xs, const void* const* ys,
+                       const double* weights, int symmetric, int accumulate, void* out, void* stream) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (!out || npairs < 0 || (npairs > 0 && (!xs || !ys || !weights))) {
+    set_error("lym_lie_derivative: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  for (int p = 0; p < npairs; ++p) {
+    if (!xs[p] || !ys[p] || xs[p] == out || ys[p] == out || !std::isfinite(weights[p])) {
+      set_error("lym_lie_derivative: null, aliased or non-finite pair " + std::to_string(p));
+      return LYM_ERR_ARG;
+    }
+  }
+  if (npairs == 0 && accumulate) return LYM_OK;
+  // launches of at most kForceMaxPairs pairs; the later ones accumulate
+  int p0 = 0;
+  do {
+    ForcePairs fp{};
+    fp.n = std::min(npairs - p0, kForceMaxPairs);
+    fp.sym = symmetric ? 1 : 0;
+    fp.accumulate = (p0 > 0 || accumulate) ? 1 : 0;
+    for (int q = 0; q < fp.n; ++q) {
+      fp.x[q] = (const cpx<double>*)xs[p0 + q];
+      fp.y[q] = (const cpx<double>*)ys[p0 + q];
+      fp.w[q] = weights[p0 + q];
+    }
+    rc = force_launch(d, fp, out, (hipStream_t)stream);
+    if (rc) return rc;
+    p0 += fp.n;
+  } while (p0 < npairs);
+  return LYM_OK;
 }
 
 int lym_gamma5(int prec, int ns, int64_t n_per_spin, const void* in, void* out, void* stream) {

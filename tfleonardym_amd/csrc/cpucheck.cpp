@@ -1,9 +1,10 @@
-// Host build of the per-site Wilson arithmetic used by the kernels
-// (wilson_site.h), looped over all sites.  Test-only: lets the CPU test suite
+// Host build of the per-site Wilson and force arithmetic used by the kernels
+// (wilson_site.h, force_site.h), looped over all sites.  Test-only: lets the CPU test suite
 // check site indexing, gamma tables, colour transport and the boundary factor
 // against the oracle without a GPU.  Not part of liblym.so.
 #include <hip/hip_runtime.h>
 
+#include "force_site.h"
 #include "wilson_site.h"
 
 using namespace lym;
@@ -29,6 +30,72 @@ static int run_mode(int proj, const Lat& lat, const double* l, const double* lr,
   if (proj) run<D, NC, ADJ, true>(lat, l, lr, in, out);
   else run<D, NC, ADJ, false>(lat, l, lr, in, out);
   return 0;
+}
+
+static Lat make_lat(int ndim, int herm, const int* dims, double kappa, double bc_re, double bc_im) {
+  Lat lat{};
+  int64_t V = 1;
+  for (int mu = 0; mu < 4; ++mu) lat.L[mu] = 1;
+  for (int mu = 0; mu < ndim; ++mu) {
+    lat.L[mu] = dims[mu];
+    V *= dims[mu];
+  }
+  int64_t st = 1;
+  for (int mu = ndim - 1; mu >= 0; --mu) {
+    lat.stride[mu] = (int)st;
+    st *= dims[mu];
+  }
+  lat.V = V;
+  lat.kappa = kappa;
+  lat.bc_re = bc_re;
+  lat.bc_im = bc_im;
+  lat.herm = herm;
+  lat.has_bc = (bc_re != 1.0 || bc_im != 0.0);
+  return lat;
+}
+
+template <int D, int NC, bool ADJ>
+static int run_force(const Lat& lat, const double* links, const double* links_r, const ForcePairs& fp,
+                     double* out) {
+  LinkView<double> lk{(const cpx<double>*)links, lat.V};
+  LinkView<double> lkr{(const cpx<double>*)(links_r ? links_r : links), lat.V};
+  for (int64_t s = 0; s < lat.V; ++s) {
+    int x[D];
+    coords<D>(lat, s, x);
+    for (int mu = 0; mu < D; ++mu) force_site<D, NC, ADJ>(lat, mu, s, x, lk, lkr, fp, 0, (cpx<double>*)out, 0);
+  }
+  return 0;
+}
+
+// out (+)= sum_p w_p [L(x_p, y_p) + sym L(y_p, x_p)], the per-site arithmetic
+// of the force kernel looped over all sites and directions
+extern "C" int lymcheck_lie_derivative(int ndim, int nc, int adj, int herm, const int* dims, double kappa,
+                                       double bc_re, double bc_im, const double* links, const double* links_r,
+                                       int npairs, const double* const* xs, const double* const* ys,
+                                       const double* weights, int sym, int accumulate, double* out) {
+  if (npairs > kForceMaxPairs) return -2;
+  const Lat lat = make_lat(ndim, herm, dims, kappa, bc_re, bc_im);
+  ForcePairs fp{};
+  fp.n = npairs;
+  fp.sym = sym;
+  fp.accumulate = accumulate;
+  for (int p = 0; p < npairs; ++p) {
+    fp.x[p] = (const cpx<double>*)xs[p];
+    fp.y[p] = (const cpx<double>*)ys[p];
+    fp.w[p] = weights[p];
+  }
+  const int key = ndim * 100 + nc * 10 + adj;
+  switch (key) {
+    case 420: return run_force<4, 2, false>(lat, links, links_r, fp, out);
+    case 430: return run_force<4, 3, false>(lat, links, links_r, fp, out);
+    case 220: return run_force<2, 2, false>(lat, links, links_r, fp, out);
+    case 230: return run_force<2, 3, false>(lat, links, links_r, fp, out);
+    case 421: return run_force<4, 2, true>(lat, links, links_r, fp, out);
+    case 431: return run_force<4, 3, true>(lat, links, links_r, fp, out);
+    case 221: return run_force<2, 2, true>(lat, links, links_r, fp, out);
+    case 231: return run_force<2, 3, true>(lat, links, links_r, fp, out);
+  }
+  return -1;
 }
 
 extern "C" int lymcheck_wilson_apply(int ndim, int nc, int adj, int herm, const int* dims, double kappa, double bc_re,

@@ -132,6 +132,10 @@ int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* o
 // out_dev[0] = max |stored - rebuilt| over the last rows of every link
 int link_deviation(const OpDesc& d, int prec, double* out_dev, hipStream_t st);
 
+// ---- fermion force (force.hip) ----------------------------------------------
+struct ForcePairs;
+int force_launch(const OpDesc& d, const ForcePairs& fp, void* out, hipStream_t st);
+
 // Largest number of blocks any reducing kernel launches for volume V
 // (Wilson blocks have at least 64 threads).
 inline int64_t max_red_blocks(int64_t V) {

@@ -276,9 +276,10 @@ __host__ __device__ __forceinline__ void wilson_hop(const Lat& lat, int64_t s, c
 // Site and coordinates of this thread under the block schedule `m`; false if
 // the thread has no site (tail of a linear schedule).  `sblk` is the site
 // block's dispatch index among `gs` site blocks (sblk % kNumXcd = its XCD).
+// `tid` / `nthr`: the thread's index among the block's `nthr` site threads.
 template <int D>
-__device__ __forceinline__ bool site_of_thread(const Lat& lat, const SiteMap& m, int sblk, int gs, int64_t& s,
-                                               int (&x)[D]) {
+__host__ __device__ __forceinline__ bool site_of(const Lat& lat, const SiteMap& m, int sblk, int gs, int tid,
+                                                 int nthr, int64_t& s, int (&x)[D]) {
   if constexpr (D == 4) {
     if (m.tiled) {
       const int xcd = sblk % kNumXcd;
@@ -288,18 +289,24 @@ __device__ __forceinline__ bool site_of_thread(const Lat& lat, const SiteMap& m,
       const int g = r % m.ng;
       x[0] = xcd * m.slab + x0l;
       x[1] = r / m.ng;
-      x[2] = g * m.k + (int)threadIdx.x / m.T;
-      x[3] = (int)threadIdx.x % m.T;
+      x[2] = g * m.k + tid / m.T;
+      x[3] = tid % m.T;
       s = (((int64_t)x[0] * lat.L[1] + x[1]) * lat.L[2] + x[2]) * m.T + x[3];
       return true;
     }
   }
   const int64_t per = gs / kNumXcd;
   const int64_t lb = (int64_t)(sblk % kNumXcd) * per + sblk / kNumXcd;
-  s = lb * blockDim.x + threadIdx.x;
+  s = lb * nthr + tid;
   if (s >= lat.V) return false;
   coords<D>(lat, s, x);
   return true;
+}
+
+template <int D>
+__device__ __forceinline__ bool site_of_thread(const Lat& lat, const SiteMap& m, int sblk, int gs, int64_t& s,
+                                               int (&x)[D]) {
+  return site_of<D>(lat, m, sblk, gs, (int)threadIdx.x, (int)blockDim.x, s, x);
 }
 
 // (site block, right-hand side) of this block in a batched launch: all RHS of

@@ -418,6 +418,54 @@ class DiracWilsonOperator(Operator):
                                          _lib.stream_handle(x.device)), "gamma5")
         return out
 
+    # ---- fermion force -------------------------------------------------------
+    def lie_derivative(self, X, Y):
+        """dirac.py:525-565: the (D, Nc, Nc, V) field kappa sum_k T^k l_k(X, Y),
+        the derivative of X^dag Op Y along U_mu(x) -> exp(i eps T^k) U_mu(x)
+        (the X^dag term; n_flavor symmetrises).  Adjoint spinors are the
+        (Ns, Nc, Nc, V) matrices; the result is in fundamental generators."""
+        return self.force_sum([(X, Y)], [1.0], symmetric=False)
+
+    def force_sum(self, pairs, weights, symmetric=True, out=None):
+        """sum_p w_p [L(x_p, y_p) + L(y_p, x_p)] (symmetric) or sum_p w_p L(x_p, y_p)
+        over (x_p, y_p) in `pairs`, in one force launch per 32 pairs
+        (lym_lie_derivative); added to `out` when given, else a new tensor.
+        This is the sum n_flavor.force forms (fermion_action.py:86-91)."""
+        shape = tuple(self.spinor_shape)
+        xs, ys = [], []
+        for x, y in pairs:
+            xd = to_device(x)
+            yd = to_device(y, device=xd.device)
+            if tuple(xd.shape) != shape or tuple(yd.shape) != shape:
+                raise RuntimeError(f"force vectors must have shape {shape}, got {tuple(xd.shape)}, {tuple(yd.shape)}")
+            xs.append(xd)
+            ys.append(yd)
+        w = []
+        for a in weights:
+            c = complex(a)
+            if c.imag != 0:
+                raise RuntimeError("force weights must be real")
+            w.append(c.real)
+        if len(w) != len(xs):
+            raise RuntimeError("one weight per pair")
+        fshape = (self.number_of_dimensions, self.number_of_colors, self.number_of_colors, self.volume)
+        dev = self._links.device
+        if out is None:
+            out = torch.empty(fshape, dtype=torch.complex128, device=dev)
+            accumulate = 0
+        else:
+            if tuple(out.shape) != fshape or out.dtype != torch.complex128 or not out.is_contiguous():
+                raise RuntimeError(f"force output must be a contiguous complex128 tensor of shape {fshape}")
+            accumulate = 1
+        n = len(xs)
+        xp = (ctypes.c_void_p * max(n, 1))(*[t.data_ptr() for t in xs])
+        yp = (ctypes.c_void_p * max(n, 1))(*[t.data_ptr() for t in ys])
+        wp = (ctypes.c_double * max(n, 1))(*w)
+        _lib.check(_lib.lib().lym_lie_derivative(ctypes.byref(self._desc), n, xp, yp, wp, 1 if symmetric else 0,
+                                                 accumulate, _lib.ptr(out), _lib.stream_handle(dev)),
+                   "lie_derivative")
+        return out
+
     def apply_square(self, vector, shift=0.0):
         """Op(Op(v)) + shift v in two fused launches (SquareOperator /
         ShiftedOperator fast path)."""
