@@ -233,6 +233,55 @@ def time_dslash(op, src, reps):
     return res
 
 
+def bytes_per_site_force(nc, ndim, npairs, link_rows=None):
+    """Compulsory bytes of one force launch per site (complex128): links read
+    once (link_rows of nc complex per direction), the (D, nc, nc) output
+    written once, X_p and Y_p of every pair read once."""
+    ns = 4 if ndim == 4 else 2
+    rows = nc if link_rows is None else link_rows
+    return ndim * rows * nc * 16 + ndim * nc * nc * 16 + npairs * 2 * ns * nc * 16
+
+
+def time_force(op, reps, npairs=4, seed=7):
+    """Average launch time of the symmetrised force sum over `npairs` pairs
+    (x_p, Q x_p) with the force_level1 rational weights (one
+    lym_lie_derivative launch), HIP events on the launch stream."""
+    import ctypes
+    import torch
+    from tfleonardym_amd import _lib, approximations
+    desc = op.desc
+    nc, V, D = op.number_of_colors, op.volume, op.number_of_dimensions
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    xs = [torch.randn(op.spinor_shape, dtype=torch.complex128, generator=g).cuda() for _ in range(npairs)]
+    ys = [op.apply(x) for x in xs]
+    al, _ = approximations.split(approximations.force_level1_rational_approximation_coefficients)
+    w = [al[p % len(al)] for p in range(npairs)]
+    out = torch.empty((D, nc, nc, V), dtype=torch.complex128, device="cuda")
+    xp = (ctypes.c_void_p * npairs)(*[t.data_ptr() for t in xs])
+    yp = (ctypes.c_void_p * npairs)(*[t.data_ptr() for t in ys])
+    wp = (ctypes.c_double * npairs)(*w)
+    stream = torch.cuda.current_stream()
+
+    def launch():
+        _lib.check(_lib.lib().lym_lie_derivative(ctypes.byref(desc), npairs, xp, yp, wp, 1, 0, _lib.ptr(out),
+                                                 _lib.stream_handle()))
+    for _ in range(3):
+        launch()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        launch()
+    e1.record(stream)
+    e1.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / reps
+    rows = nc - 1 if desc.link_format == _lib.LINKS_SU else nc
+    bps = bytes_per_site_force(nc, D, npairs, rows)
+    gbs = bps * V / (us * 1e-6) / 1e9
+    return dict(us=us, npairs=npairs, bytes_per_site=bps, gbs=gbs, frac=gbs / HBM_PEAK_GBS,
+                kernel="k_force (complex128, symmetrised, force_level1 weights)")
+
+
 def dslash_c3(reps, geometry=None, autotune=True):
     """D at 24^3 x 48 (seeded near-identity links, Gaussian input); another
     4D geometry with the same generator if given."""
@@ -248,6 +297,11 @@ def dslash_c3(reps, geometry=None, autotune=True):
     tiles = op.autotune() if autotune else {n: op.tile_info(n) for n in ("f64", "f32")}
     res = time_dslash(op, src, reps)
     res["tiles"] = tiles
+    res["force"] = time_force(op, max(5, reps // 5))
+    res["force"]["traffic"] = measured_force_traffic(g)
+    print(f"[bench] force {g} {res['force']['npairs']} pairs: {res['force']['us']:.1f} us/launch, "
+          f"{res['force']['gbs']:.0f} GB/s ({100 * res['force']['frac']:.1f}% of {HBM_PEAK_GBS:.0f})",
+          file=sys.stderr, flush=True)
     for name in ("f64", "f32"):
         res[name]["traffic"] = measured_traffic(g, name)
     res["geometry"] = list(g)
@@ -271,6 +325,21 @@ def measured_traffic(geometry, prec):
         m = re.search(r"traffic_bytes_per_launch (\S+) (f32|f64) = (\d+)", line)
         if m and m.group(1) == key and m.group(2) == prec:
             return int(m.group(3))
+    return None
+
+
+def measured_force_traffic(geometry):
+    """HBM bytes per force launch at `geometry` from the committed rocprofv3
+    summary (calibrated FETCH_SIZE + WRITE_SIZE), or None."""
+    path = os.path.join(ROOT, "profiles", "force_traffic.md")
+    if not os.path.exists(path):
+        return None
+    key = "x".join(str(L) for L in geometry)
+    import re
+    for line in open(path):
+        m = re.search(r"traffic_bytes_per_launch (\S+) force = (\d+)", line)
+        if m and m.group(1) == key:
+            return int(m.group(2))
     return None
 
 

@@ -54,17 +54,25 @@ static Lat make_lat(int ndim, int herm, const int* dims, double kappa, double bc
   return lat;
 }
 
-template <int D, int NC, bool ADJ>
-static int run_force(const Lat& lat, const double* links, const double* links_r, const ForcePairs& fp,
-                     double* out) {
+template <int D, int NC, bool ADJ, bool HERM>
+static int run_force_h(const Lat& lat, const double* links, const double* links_r, const ForcePairs& fp,
+                       double* out) {
   LinkView<double> lk{(const cpx<double>*)links, lat.V};
   LinkView<double> lkr{(const cpx<double>*)(links_r ? links_r : links), lat.V};
   for (int64_t s = 0; s < lat.V; ++s) {
     int x[D];
     coords<D>(lat, s, x);
-    for (int mu = 0; mu < D; ++mu) force_site<D, NC, ADJ>(lat, mu, s, x, lk, lkr, fp, 0, (cpx<double>*)out, 0);
+    for (int mu = 0; mu < D; ++mu)
+      force_site<D, NC, ADJ, HERM>(lat, mu, s, x, lk, lkr, fp, 0, (cpx<double>*)out, 0);
   }
   return 0;
+}
+
+template <int D, int NC, bool ADJ>
+static int run_force(const Lat& lat, const double* links, const double* links_r, const ForcePairs& fp,
+                     double* out) {
+  if (ADJ || !lat.herm) return run_force_h<D, NC, ADJ, false>(lat, links, links_r, fp, out);
+  return run_force_h<D, NC, ADJ, true>(lat, links, links_r, fp, out);
 }
 
 // out (+)= sum_p w_p [L(x_p, y_p) + sym L(y_p, x_p)], the per-site arithmetic

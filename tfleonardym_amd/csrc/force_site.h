@@ -15,6 +15,13 @@
 // links V_ab with x^a = 2 tr(T^a X): the commutators drop the trace parts of
 // X and Y exactly as that basis change does.
 //
+// The fundamental sum factorises around the link: with y_n = Y(x+mu)_n,
+//   W = U P - Q U^dag,  P = sum_n y_n (x) conj(A_n),  Q = sum_n B_n (x) conj(y_n)
+// (times the boundary factor bc on U and conj(bc) on U^dag), so the pair loop
+// only accumulates P and Q and the link is applied once per (site, mu).  When
+// hermitian, P+^dag = P- (g5 anticommutes with g_mu), so A = B, Q = P^dag and
+// W = M - M^dag with M = bc U P.
+//
 // Spin structure (chiral basis, one entry i^k per row of A_mu, see Gam<D>):
 // row r < Ns/2 pairs with lo = Ns/2 + perm_r; with g5l the gamma5 sign of the
 // lower rows (-1 hermitian, +1 otherwise) and sA = -g5l:
@@ -25,50 +32,141 @@
 
 namespace lym {
 
-// acc += contribution of one spin: colour vectors (fundamental) or row-major
-// NC x NC matrices (adjoint)
-template <int NC, bool ADJ>
-__host__ __device__ __forceinline__ void force_accumulate(const cpx<double>* v, const cpx<double>* a,
-                                                          const cpx<double>* b, cpx<double> (&acc)[NC][NC]) {
-  if constexpr (!ADJ) {
-    // acc_ut += v_u conj(a_t) - b_u conj(v_t)
+// Adjoint: acc += [v, a^dag] - [b, v^dag] for one spin (row-major NC x NC
+// matrices), i.e. acc_ij += sum_k v_ik conj(a_jk) - conj(a_ki) v_kj
+//                                - b_ik conj(v_jk) + conj(v_ki) b_kj
+template <int NC>
+__host__ __device__ __forceinline__ void force_accumulate_adj(const cpx<double>* v, const cpx<double>* a,
+                                                              const cpx<double>* b, cpx<double> (&acc)[NC][NC]) {
 #pragma unroll
-    for (int u = 0; u < NC; ++u)
+  for (int i = 0; i < NC; ++i)
 #pragma unroll
-      for (int t = 0; t < NC; ++t) {
-        cfma_c(acc[u][t], v[u], a[t]);
-        acc[u][t] -= cmul_c(b[u], v[t]);
+    for (int j = 0; j < NC; ++j) {
+      cpx<double> p = czero<double>(), m = czero<double>();
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        cfma_c(p, v[i * NC + k], a[j * NC + k]);
+        cfmac(p, v[k * NC + i], b[k * NC + j]);
+        cfmac(m, a[k * NC + i], v[k * NC + j]);
+        cfma_c(m, b[i * NC + k], v[j * NC + k]);
       }
+      acc[i][j] += p - m;
+    }
+}
+
+// i^k a for a k that varies across the lanes of a wave: selects, no branch
+__host__ __device__ __forceinline__ cpx<double> ipow_sel(int k, cpx<double> a) {
+  const cpx<double> t = (k & 1) ? mk<double>(-a.im, a.re) : a;
+  return (k & 2) ? mk<double>(-t.re, -t.im) : t;
+}
+
+// Fundamental representation: P += w sum_n y_n (x) conj(A_n) and (not
+// hermitian) Q += w sum_n B_n (x) conj(y_n) for the term of L(X, Y) at site s,
+// y = Y(x+mu) read at nb.  The loads of each spin pair are ordered after the
+// previous pair's accumulation (hop_commit / hop_dep), which keeps one pair's
+// operands live at a time; other waves hide the latency.
+template <int D, int NC, bool HERM>
+__host__ __device__ __forceinline__ void force_term_fund(const Lat& lat, int mu, const cpx<double>* X,
+                                                         const cpx<double>* Y, uint32_t bytes, int64_t s, int64_t nb,
+                                                         double w, cpx<double> (&P)[NC][NC],
+                                                         cpx<double> (&Q)[NC][NC]) {
+  constexpr int NS = Gam<D>::NS;
+  constexpr int H = NS / 2;
+  // hermitian: g5l = -1, sA = +1 (A = B); otherwise g5l = +1, sA = -1
+  constexpr double g5l = HERM ? -1.0 : 1.0;
+  constexpr double sA = -g5l;
+#pragma unroll
+  for (int r = 0; r < H; ++r) {
+    // the previous spin pair's accumulation completes before these loads
+    hop_commit<NC * NC, double>(reinterpret_cast<cpx<double>(&)[NC * NC]>(P));
+    if constexpr (!HERM) hop_commit<NC * NC, double>(reinterpret_cast<cpx<double>(&)[NC * NC]>(Q));
+    int64_t sx = s, sy = nb;
+    hop_dep<double>(sx, sy, P[NC - 1][NC - 1].re);
+    const int lo = H + Gam<D>::perm(mu, r);
+    const int k = Gam<D>::k(mu, r);
+    cpx<double> xr[NC], xl[NC], yr[NC], yl[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      xr[c] = ld<cpx<double>>(X, bytes, r * NC + c, lat.V, sx);
+      xl[c] = ld<cpx<double>>(X, bytes, lo * NC + c, lat.V, sx);
+      yr[c] = ld<cpx<double>>(Y, bytes, r * NC + c, lat.V, sy);
+      yl[c] = ld<cpx<double>>(Y, bytes, lo * NC + c, lat.V, sy);
+    }
+    // spin r:  a = w (xr + sA i^k xl),  b = w (xr + i^k xl)
+    // spin lo: a = w g5l (xl + sA i^-k xr),  b = w g5l (xl + i^-k xr)
+#pragma unroll
+    for (int t = 0; t < NC; ++t) {
+      const cpx<double> tr = ipow_sel(k, xl[t]);
+      const cpx<double> tl = ipow_sel(4 - k, xr[t]);
+      const cpx<double> ar = mk<double>(w * (xr[t].re + sA * tr.re), w * (xr[t].im + sA * tr.im));
+      const double f = w * g5l;
+      const cpx<double> al = mk<double>(f * (xl[t].re + sA * tl.re), f * (xl[t].im + sA * tl.im));
+#pragma unroll
+      for (int u = 0; u < NC; ++u) {
+        cfma_c(P[u][t], yr[u], ar);
+        cfma_c(P[u][t], yl[u], al);
+      }
+      if constexpr (!HERM) {
+        const cpx<double> br = mk<double>(w * (xr[t].re + tr.re), w * (xr[t].im + tr.im));
+        const cpx<double> bl = mk<double>(f * (xl[t].re + tl.re), f * (xl[t].im + tl.im));
+#pragma unroll
+        for (int u = 0; u < NC; ++u) {
+          cfma_c(Q[t][u], br, yr[u]);
+          cfma_c(Q[t][u], bl, yl[u]);
+        }
+      }
+    }
+  }
+}
+
+// W = bc U P - conj(bc) Q U^dag  (Q = P^dag when hermitian), in place of P
+template <int NC, bool HERM>
+__host__ __device__ __forceinline__ void force_close_fund(const cpx<double> (&u)[NC][NC], bool edge,
+                                                          const cpx<double> bcf, cpx<double> (&P)[NC][NC],
+                                                          const cpx<double> (&Q)[NC][NC]) {
+  cpx<double> M[NC][NC];
+#pragma unroll
+  for (int i = 0; i < NC; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      cpx<double> a = czero<double>();
+#pragma unroll
+      for (int k = 0; k < NC; ++k) cfma(a, u[i][k], P[k][j]);
+      M[i][j] = edge ? cmul(bcf, a) : a;
+    }
+  if constexpr (HERM) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j)
+        P[i][j] = mk<double>(M[i][j].re - M[j][i].re, M[i][j].im + M[j][i].im);
   } else {
-    // acc_ij += sum_k v_ik conj(a_jk) - conj(a_ki) v_kj - b_ik conj(v_jk) + conj(v_ki) b_kj
 #pragma unroll
     for (int i = 0; i < NC; ++i)
 #pragma unroll
       for (int j = 0; j < NC; ++j) {
-        cpx<double> p = czero<double>(), m = czero<double>();
+        cpx<double> a = czero<double>();
 #pragma unroll
-        for (int k = 0; k < NC; ++k) {
-          cfma_c(p, v[i * NC + k], a[j * NC + k]);
-          cfmac(p, v[k * NC + i], b[k * NC + j]);
-          cfmac(m, a[k * NC + i], v[k * NC + j]);
-          cfma_c(m, b[i * NC + k], v[j * NC + k]);
-        }
-        acc[i][j] += p - m;
+        for (int k = 0; k < NC; ++k) cfma_c(a, Q[i][k], u[j][k]);  // (Q U^dag)_ij
+        if (edge) a = cmul_c(a, bcf);
+        P[i][j] = M[i][j] - a;
       }
   }
 }
 
-// acc += w * [term of L(X, Y)] for direction mu at site s; nb = site of x+mu,
-// edge: the hop crosses the boundary carrying the factor (bc_re, bc_im).
+// Adjoint representation: acc += w * [term of L(X, Y)] for direction mu at
+// site s; nb = site of x+mu, edge: the hop crosses the boundary carrying the
+// factor (bc_re, bc_im).
 // X, Y: (Ns, NCC, V) complex128 planes, `bytes` their size.
-template <int D, int NC, bool ADJ>
-__host__ __device__ __forceinline__ void force_term(const Lat& lat, int mu, bool edge, const cpx<double>* X,
-                                                    const cpx<double>* Y, uint32_t bytes, int64_t s, int64_t nb,
-                                                    double w, const cpx<double> (&u)[NC][NC],
-                                                    const cpx<double> (&ur)[NC][NC], cpx<double> (&acc)[NC][NC]) {
+template <int D, int NC>
+__host__ __device__ __forceinline__ void force_term_adj(const Lat& lat, int mu, bool edge, const cpx<double>* X,
+                                                        const cpx<double>* Y, uint32_t bytes, int64_t s, int64_t nb,
+                                                        double w, const cpx<double> (&u)[NC][NC],
+                                                        const cpx<double> (&ur)[NC][NC],
+                                                        cpx<double> (&acc)[NC][NC]) {
   constexpr int NS = Gam<D>::NS;
   constexpr int H = NS / 2;
-  constexpr int NCC = ADJ ? NC * NC : NC;
+  constexpr int NCC = NC * NC;
   const double g5l = lat.herm ? -1.0 : 1.0;
   const double sA = -g5l;
   const cpx<double> bcf = mk<double>(lat.bc_re, lat.bc_im);
@@ -100,12 +198,12 @@ __host__ __device__ __forceinline__ void force_term(const Lat& lat, int mu, bool
         }
         y[c] = ld<cpx<double>>(Y, bytes, n * NCC + c, lat.V, nb);
       }
-      transport<NC, ADJ, true, double>(u, ur, y, v);
+      transport<NC, true, true, double>(u, ur, y, v);
       if (edge) {
 #pragma unroll
         for (int c = 0; c < NCC; ++c) v[c] = cmul(bcf, v[c]);
       }
-      force_accumulate<NC, ADJ>(v, a, b, acc);
+      force_accumulate_adj<NC>(v, a, b, acc);
     }
   }
 }
@@ -150,7 +248,8 @@ struct ForcePairs {
 
 // The force of direction mu at site s (coordinates x): reads the links of
 // (mu, s) once and loops over the pairs; writes out planes (mu, i, j).
-template <int D, int NC, bool ADJ>
+// HERM: lat.herm (compile-time for the fundamental form).
+template <int D, int NC, bool ADJ, bool HERM>
 __host__ __device__ __forceinline__ void force_site(const Lat& lat, int mu, int64_t s, const int (&x)[D],
                                                     const LinkView<double>& lk, const LinkView<double>& lkr,
                                                     const ForcePairs& fp, uint32_t vbytes, cpx<double>* out,
@@ -159,17 +258,32 @@ __host__ __device__ __forceinline__ void force_site(const Lat& lat, int mu, int6
   const int64_t st = lat.stride[mu];
   const int64_t nb = (x[mu] == L - 1) ? s - (int64_t)(L - 1) * st : s + st;
   const bool edge = lat.has_bc && mu == D - 1 && x[mu] == L - 1;
-  cpx<double> u[NC][NC], ur[NC][NC];
-  lk.template load<NC>(mu, s, u);
-  if constexpr (ADJ) lkr.template load<NC>(mu, s, ur);
   cpx<double> acc[NC][NC];
 #pragma unroll
   for (int i = 0; i < NC; ++i)
 #pragma unroll
     for (int j = 0; j < NC; ++j) acc[i][j] = czero<double>();
-  for (int p = 0; p < fp.n; ++p) {
-    force_term<D, NC, ADJ>(lat, mu, edge, fp.x[p], fp.y[p], vbytes, s, nb, fp.w[p], u, ur, acc);
-    if (fp.sym) force_term<D, NC, ADJ>(lat, mu, edge, fp.y[p], fp.x[p], vbytes, s, nb, fp.w[p], u, ur, acc);
+  if constexpr (!ADJ) {
+    cpx<double> Q[NC][NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) Q[i][j] = czero<double>();
+    for (int p = 0; p < fp.n; ++p) {
+      force_term_fund<D, NC, HERM>(lat, mu, fp.x[p], fp.y[p], vbytes, s, nb, fp.w[p], acc, Q);
+      if (fp.sym) force_term_fund<D, NC, HERM>(lat, mu, fp.y[p], fp.x[p], vbytes, s, nb, fp.w[p], acc, Q);
+    }
+    cpx<double> u[NC][NC];
+    lk.template load<NC>(mu, s, u);
+    force_close_fund<NC, HERM>(u, edge, mk<double>(lat.bc_re, lat.bc_im), acc, Q);
+  } else {
+    cpx<double> u[NC][NC], ur[NC][NC];
+    lk.template load<NC>(mu, s, u);
+    lkr.template load<NC>(mu, s, ur);
+    for (int p = 0; p < fp.n; ++p) {
+      force_term_adj<D, NC>(lat, mu, edge, fp.x[p], fp.y[p], vbytes, s, nb, fp.w[p], u, ur, acc);
+      if (fp.sym) force_term_adj<D, NC>(lat, mu, edge, fp.y[p], fp.x[p], vbytes, s, nb, fp.w[p], u, ur, acc);
+    }
   }
   cpx<double> res[NC][NC];
   force_finish<NC, ADJ>(lat, acc, res);
