@@ -148,6 +148,17 @@ int lym_link_deviation(const lym_wilson* op, int prec, double* dev_out, void* st
 int lym_lie_derivative(const lym_wilson* op, int npairs, const void* const* xs, const void* const* ys,
                        const double* weights, int symmetric, int accumulate, void* out, void* stream);
 
+/* ---- measurements -------------------------------------------------------
+ * out[z] (host, L_dir doubles) = sum over the nvec complex128 spinors v
+ * (op's layout) and the sites x with x_dir = z of
+ *   sum_{s,c} |v_sc(x)|^2                    (fundamental)
+ *   sum_{s,c,d} conj(v_scd(x)) v_sdc(x)      (adjoint matrices; real)
+ * the per-slice sums pion_correlator forms from its propagators
+ * (fermion_measurements.py:73-84).  scratch: device, >= (V + L_dir) doubles.
+ * Deterministic (fixed summation order).  Synchronous. */
+int lym_slice_sums(const lym_wilson* op, int dir, int nvec, const void* const* vecs, void* scratch,
+                   size_t scratch_bytes, double* out, void* stream);
+
 /* ---- kernel autotuning (no reference counterpart) ----------------------
  * The LDS-tiled Wilson kernel (4D fundamental lattices with L0 divisible
  * by 8) works on tiles of a0 x a1 x k x L3 sites with `hif` hops' loads in

@@ -467,6 +467,47 @@ int lym_lie_derivative(const lym_wilson* op, int npairs, const void* const* xs, 
   return LYM_OK;
 }
 
+int lym_slice_sums(const lym_wilson* op, int dir, int nvec, const void* const* vecs, void* scratch,
+                   size_t scratch_bytes, double* out, void* stream) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (dir < 0 || dir >= d.ndim || nvec < 0 || (nvec > 0 && !vecs) || !scratch || !out) {
+    set_error("lym_slice_sums: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  const int64_t V = d.lat.V;
+  const int L = d.lat.L[dir];
+  if (scratch_bytes < (size_t)(V + L) * sizeof(double)) {
+    set_error("lym_slice_sums: scratch smaller than (V + L_dir) doubles");
+    return LYM_ERR_ARG;
+  }
+  for (int p = 0; p < nvec; ++p) {
+    if (!vecs[p]) {
+      set_error("lym_slice_sums: null vector " + std::to_string(p));
+      return LYM_ERR_ARG;
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  double* dens = (double*)scratch;
+  double* out_dev = dens + V;
+  int p0 = 0;
+  do {
+    SliceVecs sv{};
+    sv.n = std::min(nvec - p0, kSliceMaxVec);
+    sv.accumulate = p0 > 0 ? 1 : 0;
+    for (int q = 0; q < sv.n; ++q) sv.v[q] = (const cpx<double>*)vecs[p0 + q];
+    rc = slice_density(d, sv, dens, st);
+    if (rc) return rc;
+    p0 += sv.n;
+  } while (p0 < nvec);
+  rc = slice_reduce(d, dir, dens, out_dev, st);
+  if (rc) return rc;
+  LYM_HIP(hipMemcpyAsync(out, out_dev, (size_t)L * sizeof(double), hipMemcpyDeviceToHost, st));
+  LYM_HIP(hipStreamSynchronize(st));
+  return LYM_OK;
+}
+
 int lym_gamma5(int prec, int ns, int64_t n_per_spin, const void* in, void* out, void* stream) {
   if (!in || !out || ns < 2 || (ns & 1) || n_per_spin < 0) {
     set_error("lym_gamma5: bad arguments");

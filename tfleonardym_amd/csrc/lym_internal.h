@@ -132,6 +132,18 @@ int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* o
 // out_dev[0] = max |stored - rebuilt| over the last rows of every link
 int link_deviation(const OpDesc& d, int prec, double* out_dev, hipStream_t st);
 
+// ---- measurements (measure.hip) ---------------------------------------------
+constexpr int kSliceMaxVec = 32;
+struct SliceVecs {
+  const cpx<double>* v[kSliceMaxVec];
+  int n;
+  int accumulate;
+};
+// dens[s] (+)= sum over sv of the pion density of site s (V doubles)
+int slice_density(const OpDesc& d, const SliceVecs& sv, double* dens, hipStream_t st);
+// out_dev[z] = sum of dens over the sites with x_dir = z (L_dir doubles)
+int slice_reduce(const OpDesc& d, int dir, const double* dens, double* out_dev, hipStream_t st);
+
 // ---- fermion force (force.hip) ----------------------------------------------
 struct ForcePairs;
 int force_launch(const OpDesc& d, const ForcePairs& fp, void* out, hipStream_t st);
