@@ -122,6 +122,12 @@ int lym_axpby(int prec, int64_t n, double a_re, double a_im, const void* x, doub
 /* Negate components with spin >= Ns/2 of `n_per_spin * ns` complex numbers;
  * replaces apply_hermiticity / apply_gammad (dirac.py:649-662, 686-696). */
 int lym_gamma5(int prec, int ns, int64_t n_per_spin, const void* in, void* out, void* stream);
+/* out = a x + b y over `n_per_spin * ns` complex128 numbers, with complex
+ * coefficients that differ between spins < ns/2 and the others:
+ * coef = {a_up re, im, a_lo re, im, b_up re, im, b_lo re, im}.  Replaces the
+ * per-spin combination of Overlap.apply (dirac.py:120-141). */
+int lym_spin_axpby(int ns, int64_t n_per_spin, const double* coef, const void* x, const void* y, void* out,
+                   void* stream);
 /* Precision conversion of n complex numbers (F64 <-> F32). No reference
  * counterpart: the reference runs in complex128 only. */
 int lym_convert(int prec_in, const void* in, int prec_out, void* out, int64_t n, void* stream);
@@ -147,6 +153,16 @@ int lym_link_deviation(const lym_wilson* op, int prec, double* dev_out, void* st
  * no entry may alias out.  F64 only.  Asynchronous on `stream`. */
 int lym_lie_derivative(const lym_wilson* op, int npairs, const void* const* xs, const void* const* ys,
                        const double* weights, int symmetric, int accumulate, void* out, void* stream);
+
+/* ---- gauge-field files --------------------------------------------------
+ * Unpack a gauge field stored as U* (the conjugate of U) with two columns
+ * per SU(3) link in XDR floats (big-endian IEEE single) in the order
+ * (site, mu, row, col < 2, re/im), the words read_config reads
+ * (lattice.py:10-70): conjugate, normalise column 0, orthogonalise column 1
+ * against it and normalise, column 2 = conj(col0 x col1), written to `links`
+ * ((ndim, 3, 3, volume) complex128).  raw: device copy of the
+ * volume * ndim * 12 words exactly as in the file.  Asynchronous. */
+int lym_unpack_ustar(const void* raw, int64_t volume, int ndim, void* links, void* stream);
 
 /* ---- measurements -------------------------------------------------------
  * out[z] (host, L_dir doubles) = sum over the nvec complex128 spinors v

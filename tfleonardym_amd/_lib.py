@@ -100,12 +100,14 @@ SIGNATURES = {
     "lym_dot": (_i32, [_i32, _i64, _vp, _vp, ctypes.POINTER(_f64), _vp, _vp]),
     "lym_axpby": (_i32, [_i32, _i64, _f64, _f64, _vp, _f64, _f64, _vp, _vp]),
     "lym_gamma5": (_i32, [_i32, _i32, _i64, _vp, _vp, _vp]),
+    "lym_spin_axpby": (_i32, [_i32, _i64, ctypes.POINTER(_f64), _vp, _vp, _vp, _vp]),
     "lym_convert": (_i32, [_i32, _vp, _i32, _vp, _i64, _vp]),
     "lym_link_deviation": (_i32, [_vp, _i32, _vp, _vp]),
     "lym_lie_derivative": (_i32, [ctypes.POINTER(LymWilson), _i32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                   ctypes.POINTER(_f64), _i32, _i32, _vp, _vp]),
     "lym_slice_sums": (_i32, [ctypes.POINTER(LymWilson), _i32, _i32, ctypes.POINTER(_vp), _vp, _sz,
                               ctypes.POINTER(_f64), _vp]),
+    "lym_unpack_ustar": (_i32, [_vp, _i64, _i32, _vp, _vp]),
     "lym_tune_wilson": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _vp]),
     "lym_tile_info": (_i32, [_vp, _i32, _vp]),
     "lym_solver_workspace_bytes": (_sz, [ctypes.POINTER(LymWilson), _i32, _i32]),

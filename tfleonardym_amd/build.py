@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblym.so")
 CHECK_LIB = os.path.join(HERE, "liblym_cpucheck.so")
 
-SOURCES = ["api.hip", "wilson.hip", "blas.hip", "solvers.hip", "force.hip", "measure.hip"]
+SOURCES = ["api.hip", "wilson.hip", "blas.hip", "solvers.hip", "force.hip", "measure.hip", "gauge_io.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -508,6 +508,15 @@ int lym_slice_sums(const lym_wilson* op, int dir, int nvec, const void* const* v
   return LYM_OK;
 }
 
+int lym_unpack_ustar(const void* raw, int64_t volume, int ndim, void* links, void* stream) {
+  if (!raw || !links || volume < 1 || ndim < 1 || ndim > 4 || volume > ((int64_t)1 << 31) ||
+      volume * ndim * 9 * 16 >= ((int64_t)1 << 40)) {
+    set_error("lym_unpack_ustar: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  return unpack_ustar(raw, volume, ndim, links, (hipStream_t)stream);
+}
+
 int lym_gamma5(int prec, int ns, int64_t n_per_spin, const void* in, void* out, void* stream) {
   if (!in || !out || ns < 2 || (ns & 1) || n_per_spin < 0) {
     set_error("lym_gamma5: bad arguments");
@@ -515,6 +524,18 @@ int lym_gamma5(int prec, int ns, int64_t n_per_spin, const void* in, void* out, 
   }
   if (n_per_spin == 0) return LYM_OK;
   return blas_gamma5(prec, ns, n_per_spin, in, out, (hipStream_t)stream);
+}
+
+int lym_spin_axpby(int ns, int64_t n_per_spin, const double* coef, const void* x, const void* y, void* out,
+                   void* stream) {
+  if (!coef || !x || !y || !out || ns < 2 || (ns & 1) || n_per_spin < 0) {
+    set_error("lym_spin_axpby: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  if (n_per_spin == 0) return LYM_OK;
+  SpinCoef c;
+  for (int i = 0; i < 8; ++i) c.v[i] = coef[i];
+  return blas_spin_axpby(ns, n_per_spin, c, x, y, out, (hipStream_t)stream);
 }
 
 int lym_convert(int prec_in, const void* in, int prec_out, void* out, int64_t n, void* stream) {

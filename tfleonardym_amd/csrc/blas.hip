@@ -95,6 +95,20 @@ __global__ __launch_bounds__(kBlock) void k_gamma5(int ns, int64_t nps, const E*
   }
 }
 
+// out = a_h x + b_h y with (a_h, b_h) = (c[0..1], c[4..5]) on spins < ns/2
+// and (c[2..3], c[6..7]) on the others (complex128)
+__global__ __launch_bounds__(kBlock) void k_spin_axpby(int ns, int64_t nps, SpinCoef c, const double2* __restrict__ x,
+                                                       const double2* __restrict__ y, double2* __restrict__ out) {
+  const int64_t n = (int64_t)ns * nps;
+  const int64_t half = (int64_t)(ns / 2) * nps;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int h = i >= half ? 1 : 0;
+    const double ar = c.v[2 * h], ai = c.v[2 * h + 1], br = c.v[4 + 2 * h], bi = c.v[5 + 2 * h];
+    const double2 a = x[i], b = y[i];
+    out[i] = make_double2(ar * a.x - ai * a.y + br * b.x - bi * b.y, ar * a.y + ai * a.x + br * b.y + bi * b.x);
+  }
+}
+
 static int stream_grid(int64_t n) {
   int64_t g = (n + kBlock - 1) / kBlock;
   if (g > 8192) g = 8192;
@@ -176,6 +190,15 @@ int blas_gamma5(int prec, int ns, int64_t nps, const void* in, void* out, hipStr
     set_error("lym_gamma5: precision must be F64 or F32");
     return LYM_ERR_ARG;
   }
+  LYM_HIP(hipGetLastError());
+  return LYM_OK;
+}
+
+int blas_spin_axpby(int ns, int64_t nps, const SpinCoef& c, const void* x, const void* y, void* out,
+                    hipStream_t st) {
+  const int g = stream_grid((int64_t)ns * nps);
+  hipLaunchKernelGGL(k_spin_axpby, dim3(g), dim3(kBlock), 0, st, ns, nps, c, (const double2*)x, (const double2*)y,
+                     (double2*)out);
   LYM_HIP(hipGetLastError());
   return LYM_OK;
 }

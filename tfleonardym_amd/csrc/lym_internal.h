@@ -127,6 +127,12 @@ int blas_axpby(int prec, int64_t n, double ar, double ai, const void* x, double 
                hipStream_t st);
 int blas_convert(int pin, const void* in, int pout, void* out, int64_t n, hipStream_t st);
 int blas_gamma5(int prec, int ns, int64_t nps, const void* in, void* out, hipStream_t st);
+// out = a x + b y with coefficients per spin half: v = {a_up, a_lo, b_up, b_lo} (re, im)
+struct SpinCoef {
+  double v[8];
+};
+int blas_spin_axpby(int ns, int64_t nps, const SpinCoef& c, const void* x, const void* y, void* out,
+                    hipStream_t st);
 // out_dev[0] = sum |a-b|^2, out_dev[1] = sum |a|^2
 int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* out_dev, RedCtl rc, hipStream_t st);
 // out_dev[0] = max |stored - rebuilt| over the last rows of every link
@@ -143,6 +149,9 @@ struct SliceVecs {
 int slice_density(const OpDesc& d, const SliceVecs& sv, double* dens, hipStream_t st);
 // out_dev[z] = sum of dens over the sites with x_dir = z (L_dir doubles)
 int slice_reduce(const OpDesc& d, int dir, const double* dens, double* out_dev, hipStream_t st);
+
+// ---- gauge-field files (gauge_io.hip) ---------------------------------------
+int unpack_ustar(const void* raw, int64_t V, int ndim, void* links, hipStream_t st);
 
 // ---- fermion force (force.hip) ----------------------------------------------
 struct ForcePairs;

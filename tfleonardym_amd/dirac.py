@@ -146,6 +146,88 @@ class PolynomialApproximation(Operator):
         return len(self.roots)
 
 
+class Overlap(Operator):
+    """Overlap operator built on a Wilson operator and a polynomial
+    approximation P of (Op^2)^-1/2 (dirac.py:97-172):
+        hermitian:     O v = (1+m)/2 g5 v + (1-m)/2 Op P(Op^2) v
+        non-hermitian: O v = (1+m)/2 v + (1-m)/2 g5 Op P(Op^2) v
+    The polynomial and Op run in the device kernels; the final per-spin
+    combination is one lym_spin_axpby launch."""
+
+    def __init__(self, dirac_operator, square_root_approximation, mass, hermitian=False):
+        super().__init__(hermitian)
+        self.square_root_approximation = square_root_approximation
+        self.dirac_operator = dirac_operator
+        self.mass = mass
+        self._copy_shape()
+        self.representation = self.dirac_operator.representation
+
+    def _copy_shape(self):
+        self.number_of_dimensions = self.dirac_operator.number_of_dimensions
+        self.number_of_colors = self.dirac_operator.number_of_colors
+        self.volume = self.dirac_operator.volume
+        self.spinor_dimension = self.dirac_operator.spinor_dimension
+
+    def set_gauge_configuration(self, cfg):
+        self.square_root_approximation.set_gauge_configuration(cfg)
+        self.dirac_operator.set_gauge_configuration(cfg)
+        self._copy_shape()
+
+    def apply(self, vector):
+        v = to_device(vector)
+        output = self.dirac_operator.apply(self.square_root_approximation.apply(v))
+        a = (1 + self.mass) / 2
+        b = (1 - self.mass) / 2
+        if self.hermitian:
+            coef = (a, -a, b, b)      # upper: a v + b out, lower: -a v + b out
+        else:
+            coef = (a, a, b, -b)      # upper: a v + b out, lower: a v - b out
+        c = [complex(z) for z in coef]
+        arr = (ctypes.c_double * 8)(*[p for z in c for p in (z.real, z.imag)])
+        res = torch.empty_like(v)
+        ns = v.shape[0]
+        _lib.check(_lib.lib().lym_spin_axpby(ns, v.numel() // ns, arr, _lib.ptr(v), _lib.ptr(output), _lib.ptr(res),
+                                             _lib.stream_handle(v.device)), "spin_axpby")
+        return res
+
+    def _poly_step(self, v, root):
+        """scaling (A v - root v) with A = the polynomial's operator."""
+        s = complex(self.square_root_approximation.scaling)
+        out = self.square_root_approximation.operator.apply(v)
+        return algebra_utils.axpby(-s * complex(root), v, s, out)
+
+    def lie_derivative(self, X, Y):
+        """dirac.py:143-172: (1-m)/2 [L(X, P Y) + s sum_i (L(Op l_i, r_{i+1})
+        + L(l_i, Op r_{i+1}))], l_0 = Op X, l_{i+1} = s (A - conj(root_i)) l_i,
+        r_i = the polynomial factors after root i applied to Y; all terms in
+        one force sum."""
+        roots = list(self.square_root_approximation.roots)
+        scaling = complex(self.square_root_approximation.scaling)
+        factor = 0.5 - self.mass / 2.
+        x = to_device(X)
+        right = [to_device(Y, device=x.device)]
+        for root in reversed(roots):
+            right.append(self._poly_step(right[-1], root))
+        right = list(reversed(right))
+        left = [self.dirac_operator.apply(x)]
+        for root in roots:
+            left.append(self._poly_step(left[-1], np.conj(complex(root))))
+        pairs = [(x, right[0])]
+        weights = [factor]
+        D = self.dirac_operator
+        for i in range(len(roots)):
+            pairs.append((D.apply(left[i]), right[i + 1]))
+            pairs.append((left[i], D.apply(right[i + 1])))
+            weights += [scaling * factor, scaling * factor]
+        if isinstance(D, DiracWilsonOperator):
+            return D.force_sum(pairs, weights, symmetric=False)
+        force = None
+        for (a, b), w in zip(pairs, weights):
+            term = D.lie_derivative(a, b)
+            force = algebra_utils.axpby(w, term, 0.0, term) if force is None else algebra_utils.axpby(w, term, 1.0, force)
+        return force
+
+
 class DiracWilsonOperator(Operator):
     """Wilson-Dirac operator on SU(N) links in 2 or 4 dimensions
     (dirac.py:253-662).  ``apply`` is D, or Q = g5 D when hermitian."""
@@ -429,8 +511,9 @@ class DiracWilsonOperator(Operator):
     def force_sum(self, pairs, weights, symmetric=True, out=None):
         """sum_p w_p [L(x_p, y_p) + L(y_p, x_p)] (symmetric) or sum_p w_p L(x_p, y_p)
         over (x_p, y_p) in `pairs`, in one force launch per 32 pairs
-        (lym_lie_derivative); added to `out` when given, else a new tensor.
-        This is the sum n_flavor.force forms (fermion_action.py:86-91)."""
+        (lym_lie_derivative; complex weights: one more for the imaginary
+        parts); added to `out` when given, else a new tensor.  This is the
+        sum n_flavor.force forms (fermion_action.py:86-91)."""
         shape = tuple(self.spinor_shape)
         xs, ys = [], []
         for x, y in pairs:
@@ -440,14 +523,15 @@ class DiracWilsonOperator(Operator):
                 raise RuntimeError(f"force vectors must have shape {shape}, got {tuple(xd.shape)}, {tuple(yd.shape)}")
             xs.append(xd)
             ys.append(yd)
-        w = []
-        for a in weights:
-            c = complex(a)
-            if c.imag != 0:
-                raise RuntimeError("force weights must be real")
-            w.append(c.real)
-        if len(w) != len(xs):
+        if len(weights) != len(xs):
             raise RuntimeError("one weight per pair")
+        cw = [complex(a) for a in weights]
+        if any(c.imag != 0 for c in cw):
+            # L is real-linear: sum w L = sum Re(w) L + i sum Im(w) L
+            re = self.force_sum(list(zip(xs, ys)), [c.real for c in cw], symmetric, out)
+            im = self.force_sum(list(zip(xs, ys)), [c.imag for c in cw], symmetric)
+            return algebra_utils.axpby(1j, im, 1.0, re)
+        w = [c.real for c in cw]
         fshape = (self.number_of_dimensions, self.number_of_colors, self.number_of_colors, self.volume)
         dev = self._links.device
         if out is None:
@@ -481,7 +565,8 @@ class DiracWilsonOperator(Operator):
 
 
 class Propagator:
-    """D^-1 via the square operator: g5 D (A^-1 src) (dirac.py:665-696)."""
+    """D^-1 via the square operator: g5 D (A^-1 src); for an Overlap operator
+    g5 (sol - O_{m=0} sol) (dirac.py:665-696)."""
 
     def __init__(self, operator, inverter):
         self.operator = operator
@@ -490,7 +575,16 @@ class Propagator:
     def apply(self, source):
         square_solution = self.inverter.solve(SquareOperator(self.operator), source)
         solution = self.operator.apply(square_solution)
-        return self.apply_gammad(solution)
+        if not isinstance(self.operator, Overlap):
+            return self.apply_gammad(solution)
+        # overlap: subtract the massless operator (dirac.py:675-683)
+        mass = self.operator.mass
+        self.operator.mass = 0
+        try:
+            massless = self.operator.apply(solution)
+        finally:
+            self.operator.mass = mass
+        return self.apply_gammad(algebra_utils.axpby(-1.0, massless, 1.0, solution))
 
     def apply_gammad(self, input_vector):
         x = to_device(input_vector)
