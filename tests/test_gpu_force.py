@@ -112,7 +112,7 @@ def test_force_sum_many_pairs(geometry, nc):
     assert not bool(zero.abs().max().item())
 
 
-def test_force_shape_and_weight_errors():
+def test_force_shape_errors_and_complex_weights():
     geometry = (4, 6)
     U = fields.gauge_field(geometry, 2, seed=1, kind="haar")
     op = make_op(U, geometry, 0.1, True, "fundamental", None)
@@ -120,9 +120,11 @@ def test_force_shape_and_weight_errors():
     with pytest.raises(RuntimeError):
         op.lie_derivative(X[:, :, :-1], Y[:, :, :-1])
     with pytest.raises(RuntimeError):
-        op.force_sum([(X, Y)], [1.0 + 1.0j])
-    with pytest.raises(RuntimeError):
         op.force_sum([(X, Y)], [1.0, 2.0])
+    # complex weights: sum Re(w) L + i sum Im(w) L
+    L = op.lie_derivative(X, Y).cpu().numpy()
+    got = op.force_sum([(X, Y)], [0.5 + 2.0j], symmetric=False).cpu().numpy()
+    assert np.abs(got - (0.5 + 2.0j) * L).max() < 1e-13 * np.abs(L).max()
 
 
 def test_force_full_size_properties():
