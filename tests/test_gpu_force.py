@@ -69,6 +69,29 @@ def test_lie_derivative_matches_oracle(geometry, nc, rep, herm, bc):
     assert rel(got, ofo.lie_derivative(wo, X, Y)) < TOL64
 
 
+@pytest.mark.parametrize("geometry", [(8, 2, 8, 64), (8, 4, 8, 8), (16, 2, 4, 16)])
+@pytest.mark.parametrize("nc", [2, 3])
+@pytest.mark.parametrize("herm", [True, False])
+@pytest.mark.parametrize("lds", ["0", "1"])
+def test_force_tiled_schedules_match_oracle(geometry, nc, herm, lds, monkeypatch):
+    """Lattices on the XCD-tiled schedule: the unstaged kernel and the opt-in
+    LDS-staged one (LYM_FORCE_LDS=1, hermitian; tile faces in x2 read
+    globally when the block does not span x2, in-tile wrap when it does),
+    3 symmetrised weighted pairs."""
+    print(f"case force tiled geometry={geometry} nc={nc} herm={herm} lds={lds}", flush=True)
+    monkeypatch.setenv("LYM_FORCE_LDS", lds)
+    U = fields.gauge_field(geometry, nc, seed=47, kind="haar")
+    wo = ow.WilsonOracle(U, geometry, 0.125, herm, "fundamental", bc_factor=-1.0)
+    op = make_op(U, geometry, 0.125, herm, "fundamental", -1.0)
+    vs = _vecs(geometry, nc, "fundamental", 6, n=4)
+    pairs = [(vs[0], vs[1]), (vs[2], vs[3]), (vs[1], vs[2])]
+    weights = [0.4, -1.1, 2.3]
+    want = sum(w * (ofo.lie_derivative(wo, a, b) + ofo.lie_derivative(wo, b, a)) for (a, b), w in zip(pairs, weights))
+    assert rel(op.force_sum(pairs, weights, symmetric=True), want) < TOL64
+    want1 = ofo.lie_derivative(wo, vs[0], vs[3])
+    assert rel(op.lie_derivative(vs[0], vs[3]), want1) < TOL64
+
+
 @pytest.mark.parametrize("rep,nc", [("fundamental", 3), ("adjoint", 2), ("adjoint", 3)])
 def test_lie_derivative_custom_boundary_provider(rep, nc):
     """A provider other than the standard one: links signed on upload (and

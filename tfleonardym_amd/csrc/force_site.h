@@ -60,16 +60,47 @@ __host__ __device__ __forceinline__ cpx<double> ipow_sel(int k, cpx<double> a) {
   return (k & 2) ? mk<double>(-t.re, -t.im) : t;
 }
 
+// Component accessors of one spinor at one site: (plane) -> value.
+// GlobalAcc reads device memory (buffer loads); `order(dep)` routes its site
+// index through an empty asm that consumes dep, so its loads issue after dep
+// is produced.  LdsAcc reads a staged tile (plane-major, nsite per plane);
+// MixAcc picks one of the two per lane.
+struct GlobalAcc {
+  const cpx<double>* p;
+  uint32_t bytes;
+  int64_t V;
+  int64_t s;
+  __host__ __device__ __forceinline__ cpx<double> operator()(int plane) const {
+    return ld<cpx<double>>(p, bytes, plane, V, s);
+  }
+  __host__ __device__ __forceinline__ void order(double dep) {
+    int64_t t = 0;
+    hop_dep<double>(s, t, dep);
+  }
+};
+struct LdsAcc {
+  const cpx<double>* base;
+  int nsite;
+  int sl;
+  __host__ __device__ __forceinline__ cpx<double> operator()(int plane) const { return base[plane * nsite + sl]; }
+  __host__ __device__ __forceinline__ void order(double) {}
+};
+struct MixAcc {
+  LdsAcc l;
+  GlobalAcc g;
+  bool lds;
+  __host__ __device__ __forceinline__ cpx<double> operator()(int plane) const { return lds ? l(plane) : g(plane); }
+  __host__ __device__ __forceinline__ void order(double dep) { g.order(dep); }
+};
+
 // Fundamental representation: P += w sum_n y_n (x) conj(A_n) and (not
-// hermitian) Q += w sum_n B_n (x) conj(y_n) for the term of L(X, Y) at site s,
-// y = Y(x+mu) read at nb.  The loads of each spin pair are ordered after the
-// previous pair's accumulation (hop_commit / hop_dep), which keeps one pair's
-// operands live at a time; other waves hide the latency.
-template <int D, int NC, bool HERM>
-__host__ __device__ __forceinline__ void force_term_fund(const Lat& lat, int mu, const cpx<double>* X,
-                                                         const cpx<double>* Y, uint32_t bytes, int64_t s, int64_t nb,
-                                                         double w, cpx<double> (&P)[NC][NC],
-                                                         cpx<double> (&Q)[NC][NC]) {
+// hermitian) Q += w sum_n B_n (x) conj(y_n) for the term of L(X, Y) at one
+// site, xa = X(x), ya = Y(x+mu).  The loads of each spin pair are ordered
+// after the previous pair's accumulation (hop_commit / order), which keeps
+// one pair's operands live at a time; other waves hide the latency.
+template <int D, int NC, bool HERM, class XA, class YA>
+__host__ __device__ __forceinline__ void force_term_acc(int mu, XA xa, YA ya, double w, cpx<double> (&P)[NC][NC],
+                                                        cpx<double> (&Q)[NC][NC]) {
   constexpr int NS = Gam<D>::NS;
   constexpr int H = NS / 2;
   // hermitian: g5l = -1, sA = +1 (A = B); otherwise g5l = +1, sA = -1
@@ -80,17 +111,17 @@ __host__ __device__ __forceinline__ void force_term_fund(const Lat& lat, int mu,
     // the previous spin pair's accumulation completes before these loads
     hop_commit<NC * NC, double>(reinterpret_cast<cpx<double>(&)[NC * NC]>(P));
     if constexpr (!HERM) hop_commit<NC * NC, double>(reinterpret_cast<cpx<double>(&)[NC * NC]>(Q));
-    int64_t sx = s, sy = nb;
-    hop_dep<double>(sx, sy, P[NC - 1][NC - 1].re);
+    xa.order(P[NC - 1][NC - 1].re);
+    ya.order(P[NC - 1][NC - 1].re);
     const int lo = H + Gam<D>::perm(mu, r);
     const int k = Gam<D>::k(mu, r);
     cpx<double> xr[NC], xl[NC], yr[NC], yl[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      xr[c] = ld<cpx<double>>(X, bytes, r * NC + c, lat.V, sx);
-      xl[c] = ld<cpx<double>>(X, bytes, lo * NC + c, lat.V, sx);
-      yr[c] = ld<cpx<double>>(Y, bytes, r * NC + c, lat.V, sy);
-      yl[c] = ld<cpx<double>>(Y, bytes, lo * NC + c, lat.V, sy);
+      xr[c] = xa(r * NC + c);
+      xl[c] = xa(lo * NC + c);
+      yr[c] = ya(r * NC + c);
+      yl[c] = ya(lo * NC + c);
     }
     // spin r:  a = w (xr + sA i^k xl),  b = w (xr + i^k xl)
     // spin lo: a = w g5l (xl + sA i^-k xr),  b = w g5l (xl + i^-k xr)
@@ -117,6 +148,15 @@ __host__ __device__ __forceinline__ void force_term_fund(const Lat& lat, int mu,
       }
     }
   }
+}
+
+// the same with X(x) and Y(x+mu) read from device memory at s and nb
+template <int D, int NC, bool HERM>
+__host__ __device__ __forceinline__ void force_term_fund(const Lat& lat, int mu, const cpx<double>* X,
+                                                         const cpx<double>* Y, uint32_t bytes, int64_t s, int64_t nb,
+                                                         double w, cpx<double> (&P)[NC][NC],
+                                                         cpx<double> (&Q)[NC][NC]) {
+  force_term_acc<D, NC, HERM>(mu, GlobalAcc{X, bytes, lat.V, s}, GlobalAcc{Y, bytes, lat.V, nb}, w, P, Q);
 }
 
 // W = bc U P - conj(bc) Q U^dag  (Q = P^dag when hermitian), in place of P
