@@ -30,8 +30,9 @@ int64_t field_elems(const OpDesc& d) { return (int64_t)d.ncomp * d.lat.V; }
 
 // Every LDS tile a 4D fundamental lattice admits (L0 divisible by the XCD
 // count): a0 | L0/8, a1 | L1, k | L2, a0*a1*k*L3 a multiple of 64 and at
-// most max_threads sites, lds_per_site * sites within the LDS budget.
-static std::vector<TileMap> admissible_tiles(const lym_wilson* w, int adj, int64_t lds_per_site, int max_threads) {
+// most max_threads sites, lds_per_site * sites within max_lds bytes.
+static std::vector<TileMap> admissible_tiles(const lym_wilson* w, int adj, int64_t lds_per_site, int max_threads,
+                                             int64_t max_lds) {
   std::vector<TileMap> out;
   if (w->ndim != 4 || adj || w->dims[0] % kNumXcd) return out;
   const int L0 = w->dims[0], L1 = w->dims[1], L2 = w->dims[2], T = w->dims[3];
@@ -44,7 +45,7 @@ static std::vector<TileMap> admissible_tiles(const lym_wilson* w, int adj, int64
       for (int k = 1; k <= L2; ++k) {
         if (L2 % k) continue;
         const int64_t ns = (int64_t)a0 * a1 * k * T;
-        if (ns % 64 || ns > max_threads || ns * lds_per_site > kTileMaxLds) continue;
+        if (ns % 64 || ns > max_threads || ns * lds_per_site > max_lds) continue;
         out.push_back(TileMap{1, a0, a1, k, T, slab, slab / a0, L1 / a1, L2 / k, (int)ns, (int)(V / ns), 1});
       }
     }
@@ -167,9 +168,9 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   // (smaller tiles re-read more, larger ones lose occupancy); ties go to the
   // deeper x2 extent (neighbours nearer in memory), then the smaller tile.
   // lds_per_site: LDS bytes a tile site needs; [lo, hi]: preferred tile sizes
-  auto choose_tile = [&](int64_t lds_per_site, int max_threads, int lo, int hi) {
+  auto choose_tile = [&](int64_t lds_per_site, int max_threads, int lo, int hi, int64_t max_lds) {
     TileMap best{};
-    const std::vector<TileMap> all = admissible_tiles(w, o.adj, lds_per_site, max_threads);
+    const std::vector<TileMap> all = admissible_tiles(w, o.adj, lds_per_site, max_threads, max_lds);
     const int L0 = w->dims[0], L1 = w->dims[1], L2 = w->dims[2];
     auto part = [](int a, int L) { return a == L ? 0.0 : 2.0 / a; };
     for (int pass = 0; pass < 2 && !best.on; ++pass) {
@@ -191,7 +192,7 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   const int want_hif = hf ? std::atoi(hf) : 0;
   for (int pi = 0; pi < 2; ++pi) {
     const int64_t es = pi == 0 ? 16 : 8;
-    TileMap best = choose_tile(o.ncomp * es, pi == 0 ? kTileMaxThreads64 : kTileMaxThreads, 256, 512);
+    TileMap best = choose_tile(o.ncomp * es, pi == 0 ? kTileMaxThreads64 : kTileMaxThreads, 256, 512, kTileMaxLds);
     o.tile_tuned[pi] = 0;
     if (!want[0] && !want_hif && o.use_tile) {
       // a shape recorded by lym_tune_wilson for this lattice wins
@@ -209,7 +210,7 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   // (LYM_MR=1 in the environment): kMrRhs spinor tiles leave LDS room for
   // only a few waves per CU, and measured on MI355X that costs more than the
   // saved link traffic (batched CG, 16^3x32 and 24^3x48: 8% and 17% slower).
-  o.tile_mr = choose_tile(o.ncomp * 8 * kMrRhs, kTileMaxThreads, 128, 256);
+  o.tile_mr = choose_tile(o.ncomp * 8 * kMrRhs, kTileMaxThreads, 128, 256, kTileMaxLdsMr);
   const char* mr = std::getenv("LYM_MR");
   o.use_mr = (mr && mr[0] == '1') ? 1 : 0;
   *d = o;
@@ -366,10 +367,11 @@ int lym_tune_wilson(const lym_wilson* op, int prec, int reps, const void* in, vo
   // candidates: the untiled kernel, then every tile x hops in flight
   std::vector<TileMap> cands{TileMap{}};
   for (const TileMap& t : admissible_tiles(op, d.adj, (int64_t)d.ncomp * (pi == 0 ? 16 : 8),
-                                           pi == 0 ? kTileMaxThreads64 : kTileMaxThreads)) {
+                                           pi == 0 ? kTileMaxThreads64 : kTileMaxThreads, kTileMaxLds)) {
     for (int hif : {1, 2, 4}) {
       if (hif == 4 && prec != LYM_PREC_F32) continue;
-      if (hif > 1 && t.nsite > 512) continue;  // the launcher runs those with one hop in flight
+      // the launcher runs larger tiles with one hop in flight
+      if (hif > 1 && t.nsite > tile_hif_threads(prec == LYM_PREC_F64 ? P64 : P32)) continue;
       TileMap c = t;
       c.hif = hif;
       cands.push_back(c);

@@ -164,8 +164,15 @@ inline int64_t max_red_blocks(int64_t V) {
   return g > 2048 ? g : 2048;
 }
 constexpr int kMaxRed = 4;  // most values any kernel reduces
-constexpr int kTileMaxThreads = 640;        // LDS-tiled Wilson block size limit
-constexpr int kTileMaxThreads64 = 320;      // the same for complex128 (LDS bound)
-constexpr int64_t kTileMaxLds = 64 * 1024;  // LDS bytes per tiled block
+constexpr int kTileMaxThreads = 640;    // LDS-tiled Wilson block size limit (one hop in flight)
+constexpr int kTileMaxThreads64 = 640;  // the same for complex128
+// blocks that keep several hops in flight need more registers per thread:
+constexpr int kTileHifThreads32 = 512;  // complex64 / 16-bit, HIF 2 and 4
+constexpr int kTileHifThreads64 = 320;  // complex128, HIF 2
+// LDS bytes per tiled block (gfx950 has 160 KiB per CU; launches above the
+// default 64 KiB raise the kernel's dynamic-LDS limit first)
+constexpr int64_t kTileMaxLds = 150 * 1024;
+constexpr int64_t kTileMaxLdsMr = 64 * 1024;  // multi-RHS tile
+inline int tile_hif_threads(int prec) { return prec == P64 ? kTileHifThreads64 : kTileHifThreads32; }
 
 }  // namespace lym

@@ -475,7 +475,8 @@ struct TileView {
 };
 
 template <int NC, bool ADJ, int P, class Epi, int HIF>
-__global__ __launch_bounds__(P == P64 ? kTileMaxThreads64 : (HIF == 1 ? kTileMaxThreads : 512),
+__global__ __launch_bounds__(HIF == 1 ? (P == P64 ? kTileMaxThreads64 : kTileMaxThreads)
+                                       : (P == P64 ? kTileHifThreads64 : kTileHifThreads32),
                              HIF == 1 ? 3 : 2) void k_wilson_tile(Lat lat, TileMap tm, SpinorView<P> in,
                                                                  int64_t in_stride,
                                                                  LinkView<typename Store<P>::R> lk,
@@ -738,9 +739,22 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
       const size_t lds = (size_t)NCOMP * tm.nsite * sizeof(typename Store<P>::lds_elem);
       const dim3 grid((unsigned)tm.blocks * (unsigned)nrhs), block(tm.nsite);
       // hops in flight: 4 is built for complex64 only (complex128 would
-      // spill); more than one needs blocks of at most 512 threads
+      // spill); more than one needs blocks of at most 512 (complex128: 320)
+      // threads
       int hif = (P != P32 && tm.hif == 4) ? 2 : tm.hif;
-      if (tm.nsite > 512) hif = 1;
+      if (tm.nsite > tile_hif_threads(P)) hif = 1;
+      if (lds > 64 * 1024) {
+        // above the default dynamic-LDS limit: raise it once per kernel
+        static bool raised[3] = {false, false, false};
+        const int hi = hif == 4 ? 2 : hif - 1;
+        if (!raised[hi]) {
+          const void* fn = hif == 4   ? (const void*)k_wilson_tile<NC, ADJ, P, Epi, (P == P32 ? 4 : 1)>
+                           : hif == 2 ? (const void*)k_wilson_tile<NC, ADJ, P, Epi, 2>
+                                      : (const void*)k_wilson_tile<NC, ADJ, P, Epi, 1>;
+          LYM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTileMaxLds));
+          raised[hi] = true;
+        }
+      }
       if (hif == 4) {
         if constexpr (P == P32)
           hipLaunchKernelGGL((k_wilson_tile<NC, ADJ, P, Epi, 4>), grid, block, lds, stream, d.lat, tm, sv,
