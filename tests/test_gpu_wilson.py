@@ -70,8 +70,8 @@ def test_tiled_schedule_parity(geometry, nc, rep, bc):
 
 def _tile_shapes(geometry, es, ncomp):
     """Every LDS tile (a0, a1, k) the library admits on `geometry`: a0 divides
-    L0/8, a1 | L1, k | L2, a0*a1*k*L3 a multiple of 64 and <= 640 sites,
-    tile spinor <= 150 KiB."""
+    L0/8, a1 | L1, k | L2, a0*a1*k*L3 a multiple of 64 and <= 1024 sites
+    (640 for complex128), tile spinor <= 150 KiB."""
     L0, L1, L2, T = geometry
     slab = L0 // 8
     out = []
@@ -79,7 +79,7 @@ def _tile_shapes(geometry, es, ncomp):
         for a1 in range(1, L1 + 1):
             for k in range(1, L2 + 1):
                 ns = a0 * a1 * k * T
-                if slab % a0 or L1 % a1 or L2 % k or ns % 64 or ns > 640 or ns * ncomp * es > 150 * 1024:
+                if slab % a0 or L1 % a1 or L2 % k or ns % 64 or ns > (640 if es == 16 else 1024) or ns * ncomp * es > 150 * 1024:
                     continue
                 out.append((a0, a1, k))
     return out

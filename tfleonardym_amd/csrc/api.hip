@@ -210,7 +210,7 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   // (LYM_MR=1 in the environment): kMrRhs spinor tiles leave LDS room for
   // only a few waves per CU, and measured on MI355X that costs more than the
   // saved link traffic (batched CG, 16^3x32 and 24^3x48: 8% and 17% slower).
-  o.tile_mr = choose_tile(o.ncomp * 8 * kMrRhs, kTileMaxThreads, 128, 256, kTileMaxLdsMr);
+  o.tile_mr = choose_tile(o.ncomp * 8 * kMrRhs, kTileMaxThreadsMr, 128, 256, kTileMaxLdsMr);
   const char* mr = std::getenv("LYM_MR");
   o.use_mr = (mr && mr[0] == '1') ? 1 : 0;
   *d = o;

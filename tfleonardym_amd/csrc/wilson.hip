@@ -576,7 +576,7 @@ __device__ __forceinline__ Arr<Epi, sizeof...(I)> group_epis(const Epi& epi, int
 }
 
 template <int NC, int P, class Epi>
-__global__ __launch_bounds__(kTileMaxThreads, 2) void k_wilson_tile_mr(Lat lat, TileMap tm, SpinorView<P> in,
+__global__ __launch_bounds__(kTileMaxThreadsMr, 2) void k_wilson_tile_mr(Lat lat, TileMap tm, SpinorView<P> in,
                                                                        int64_t in_stride,
                                                                        LinkView<typename Store<P>::R> lk, Epi epi,
                                                                        RedCtl rc, int nrhs, int ngroups) {
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(kTileMaxThreads, 2) void k_wilson_tile_mr(Lat lat, 
     for (int i = 0; i < NR; ++i) red[i] = 0.0;
     e.v[r].apply(lat, s, view[r], acc[r], red);
     if constexpr (Epi::NRED > 0) {
-      block_partials<NR, kTileMaxThreads / 64>(red, rc.partials + (int64_t)(rhs0 + r) * NR * tm.blocks, tblk,
+      block_partials<NR, kTileMaxThreadsMr / 64>(red, rc.partials + (int64_t)(rhs0 + r) * NR * tm.blocks, tblk,
                                                tm.blocks);
       __syncthreads();  // the reduction's LDS is reused by the next RHS
     }
