@@ -163,6 +163,12 @@ int lym_lie_derivative(const lym_wilson* op, int npairs, const void* const* xs, 
  * ((ndim, 3, 3, volume) complex128).  raw: device copy of the
  * volume * ndim * 12 words exactly as in the file.  Asynchronous. */
 int lym_unpack_ustar(const void* raw, int64_t volume, int ndim, void* links, void* stream);
+/* out = X (X^dag X)^-1/2 / det(...)^(1/nc) for every link X of `in`
+ * ((ndim, nc, nc, volume) complex128; nc 2 or 3): the unitary polar factor
+ * divided by the principal nc-th root of its determinant, as
+ * unitary_sqrt_projection (smearing.py:18-31) builds the random links of
+ * hotstart (hmc.py:16-31).  in and out may be the same.  Asynchronous. */
+int lym_unitary_projection(int ndim, int nc, int64_t volume, const void* in, void* out, void* stream);
 
 /* ---- measurements -------------------------------------------------------
  * out[z] (host, L_dir doubles) = sum over the nvec complex128 spinors v

@@ -108,6 +108,7 @@ SIGNATURES = {
     "lym_slice_sums": (_i32, [ctypes.POINTER(LymWilson), _i32, _i32, ctypes.POINTER(_vp), _vp, _sz,
                               ctypes.POINTER(_f64), _vp]),
     "lym_unpack_ustar": (_i32, [_vp, _i64, _i32, _vp, _vp]),
+    "lym_unitary_projection": (_i32, [_i32, _i32, _i64, _vp, _vp, _vp]),
     "lym_tune_wilson": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _vp]),
     "lym_tile_info": (_i32, [_vp, _i32, _vp]),
     "lym_solver_workspace_bytes": (_sz, [ctypes.POINTER(LymWilson), _i32, _i32]),

@@ -519,6 +519,14 @@ int lym_unpack_ustar(const void* raw, int64_t volume, int ndim, void* links, voi
   return unpack_ustar(raw, volume, ndim, links, (hipStream_t)stream);
 }
 
+int lym_unitary_projection(int ndim, int nc, int64_t volume, const void* in, void* out, void* stream) {
+  if (!in || !out || volume < 1 || ndim < 1 || ndim > 4 || volume > ((int64_t)1 << 31)) {
+    set_error("lym_unitary_projection: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  return unitary_project(ndim, nc, volume, in, out, (hipStream_t)stream);
+}
+
 int lym_gamma5(int prec, int ns, int64_t n_per_spin, const void* in, void* out, void* stream) {
   if (!in || !out || ns < 2 || (ns & 1) || n_per_spin < 0) {
     set_error("lym_gamma5: bad arguments");

@@ -152,6 +152,7 @@ int slice_reduce(const OpDesc& d, int dir, const double* dens, double* out_dev, 
 
 // ---- gauge-field files (gauge_io.hip) ---------------------------------------
 int unpack_ustar(const void* raw, int64_t V, int ndim, void* links, hipStream_t st);
+int unitary_project(int ndim, int nc, int64_t V, const void* in, void* out, hipStream_t st);
 
 // ---- fermion force (force.hip) ----------------------------------------------
 struct ForcePairs;
