@@ -199,6 +199,29 @@ def test_mixed_cg_max_steps():
     assert s.last_status == "max_iterations"
 
 
+@pytest.mark.parametrize("mode,shifts", [("mixed", [0.01]), ("mixed", [1.0, 0.1, 0.01]), ("double", [0.1, 0.01]),
+                                         ("half", [0.01])])
+def test_non_finite_source_is_a_breakdown(mode, shifts):
+    """A source holding a NaN ends the solve at once with status breakdown
+    (not a converged zero solution), alone and inside a batch whose other
+    sources still converge."""
+    from tfleonardym_amd import dirac, dirac_solver
+    geometry = (4, 4, 4, 4)
+    op, oop = setup(geometry, 3, 0.13)
+    good = fields.gaussian_spinor(geometry, 4, 3, seed=9)
+    bad = good.copy()
+    bad[1, 2, 17] = np.nan
+    s = dirac_solver.multishift_solver(2000, 0, algorithm="cg", precision_mode=mode, verbose=False)
+    s.solve(dirac.SquareOperator(op), bad, shifts)
+    assert s.last_status == "breakdown"
+    if mode != "double":
+        xs = s.solve_batch(dirac.SquareOperator(op), [good, bad, good], shifts)
+        st = s.last_stats
+        assert [x["status"] for x in st] == ["converged", "breakdown", "converged"]
+        for k, sh in enumerate(shifts):
+            assert osv.true_residual(oop.apply_square, good, xs[2][k].cpu().numpy(), sh) <= 1e-10
+
+
 def test_c1_plain_fp64_cg():
     """The CPU-runnable workload: 2D 24x16 SU(2), Haar links, kappa=-0.15,
     periodic x / antiperiodic t, FP64 CG on D^dag D to 1e-10."""

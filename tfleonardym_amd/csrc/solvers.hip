@@ -127,7 +127,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       st->tol2_true *= sums[0];
       st->rr_max = sums[0];
       if (!(sums[0] > 0.0)) {
-        st->status = ST_CONVERGED;
+        // a zero source is solved by x = 0; a non-finite one is a breakdown
+        st->status = isfinite(sums[0]) ? ST_CONVERGED : ST_BREAKDOWN;
         st->phase = PH_DONE;
         if (ctl) {  // a zero source is finished (and needs no precision switch)
           atomicAdd(&ctl->ndone, 1);
@@ -1261,12 +1262,15 @@ static void fill_stats(lym_solve_stats* s, const SolverState& h, int nshift) {
 // misses the target.  xs[r * nshift + k]; rel[r * nshift + k] receives the
 // final true relative residuals; returns the number of correction cycles.
 static int verify_and_refine(const OpDesc& d, const Parts& P, int prec, int nrhs, int nshift, const double* shifts,
-                             const void* const* bs, void* const* xs, double tol, int max_iter, bool breakdown,
-                             hipStream_t st, std::vector<double>& rel, int* restarts) {
+                             const void* const* bs, void* const* xs, double tol, int max_iter,
+                             const std::vector<SolverState>& hs, hipStream_t st, std::vector<double>& rel,
+                             int* restarts) {
   const int64_t n = field_elems(d);
   const int max_refine = 12;
   rel.assign((size_t)nrhs * nshift, 0.0);
   for (int r = 0; r < nrhs; ++r) {
+    // a right-hand side whose solve broke down is reported, not refined
+    const bool breakdown = hs[r].status == ST_BREAKDOWN;
     Field bf;
     bf.data = const_cast<void*>(bs[r]);
     for (int k = 0; k < nshift; ++k) {
@@ -1376,11 +1380,9 @@ int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const dou
       rc = cgm_fast(d, P, bs, xs, nrhs, nshift, shifts, tol_rel.data(), max_iter, 1e-4, P32, st, hs);
     if (rc) return rc;
   }
-  bool breakdown = false;
-  for (auto& h : hs) breakdown |= (h.status == ST_BREAKDOWN);
   std::vector<double> rel;
   int restarts = 0;
-  rc = verify_and_refine(d, P, prec, nrhs, nshift, shifts, bs, xs, tol, max_iter, breakdown, st, rel, &restarts);
+  rc = verify_and_refine(d, P, prec, nrhs, nshift, shifts, bs, xs, tol, max_iter, hs, st, rel, &restarts);
   if (rc) return rc;
   int worst = LYM_OK;
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
