@@ -197,17 +197,53 @@ def bytes_per_site_dslash(nc, ndim, wl, ws, link_rows=None):
 DSLASH_C3 = dict(geometry=(24, 24, 24, 48), nc=3, kappa=0.1335, eps=0.25, seed=20250)
 
 
+def kernel_time_us(launch, reps):
+    """Average duration in us of one `launch()` (a single kernel) over `reps`
+    back-to-back launches.  The launches are captured into a HIP graph and
+    replayed, so host-side launch overhead (ctypes, descriptor validation, a
+    profiler's API interception) does not enter the figure; HIP events are
+    recorded on the stream the graph replays on.  Falls back to timing the
+    direct launches if the capture fails."""
+    import torch
+    for _ in range(3):
+        launch()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    graph = None
+    try:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(reps):
+                launch()
+        graph.replay()
+        torch.cuda.synchronize()
+    except Exception as exc:  # capture not possible: direct launches
+        print(f"[bench] graph capture failed ({exc}); timing direct launches", file=sys.stderr, flush=True)
+        graph = None
+        torch.cuda.synchronize()
+    e0.record(stream)
+    if graph is not None:
+        graph.replay()
+    else:
+        for _ in range(reps):
+            launch()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) * 1e3 / reps
+
+
 def time_dslash(op, src, reps):
     """Average launch time of D (lym_wilson_apply, projected form) in
-    complex128 and complex64, timed with HIP events on the stream the
-    kernels are launched on; GB/s from the compulsory bytes per site."""
+    complex128 and complex64 (kernel_time_us: HIP events around a graph of
+    back-to-back launches); GB/s from the compulsory bytes per site."""
     import ctypes
     import torch
     from tfleonardym_amd import _lib
     desc = op.ensure_f32()
     nc, V = op.number_of_colors, op.volume
     rows = nc - 1 if desc.link_format == _lib.LINKS_SU else nc
-    stream = torch.cuda.current_stream()
     res = {}
     for prec, name, w in ((_lib.PREC_F64, "f64", 8), (_lib.PREC_F32, "f32", 4)):
         dt = torch.complex128 if prec == _lib.PREC_F64 else torch.complex64
@@ -217,16 +253,7 @@ def time_dslash(op, src, reps):
         def launch():
             _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), prec, 0, _lib.ptr(x), _lib.ptr(y),
                                                    _lib.stream_handle()))
-        for _ in range(5):
-            launch()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(reps):
-            launch()
-        e1.record(stream)
-        e1.synchronize()
-        us = e0.elapsed_time(e1) * 1e3 / reps
+        us = kernel_time_us(launch, reps)
         bps = bytes_per_site_dslash(nc, 4, w, w, rows)
         gbs = bps * V / (us * 1e-6) / 1e9
         res[name] = dict(us=us, bytes_per_site=bps, gbs=gbs, frac=gbs / HBM_PEAK_GBS)
@@ -245,7 +272,7 @@ def bytes_per_site_force(nc, ndim, npairs, link_rows=None):
 def time_force(op, reps, npairs=4, seed=7):
     """Average launch time of the symmetrised force sum over `npairs` pairs
     (x_p, Q x_p) with the force_level1 rational weights (one
-    lym_lie_derivative launch), HIP events on the launch stream."""
+    lym_lie_derivative launch), timed by kernel_time_us."""
     import ctypes
     import torch
     from tfleonardym_amd import _lib, approximations
@@ -260,21 +287,11 @@ def time_force(op, reps, npairs=4, seed=7):
     xp = (ctypes.c_void_p * npairs)(*[t.data_ptr() for t in xs])
     yp = (ctypes.c_void_p * npairs)(*[t.data_ptr() for t in ys])
     wp = (ctypes.c_double * npairs)(*w)
-    stream = torch.cuda.current_stream()
 
     def launch():
         _lib.check(_lib.lib().lym_lie_derivative(ctypes.byref(desc), npairs, xp, yp, wp, 1, 0, _lib.ptr(out),
                                                  _lib.stream_handle()))
-    for _ in range(3):
-        launch()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        launch()
-    e1.record(stream)
-    e1.synchronize()
-    us = e0.elapsed_time(e1) * 1e3 / reps
+    us = kernel_time_us(launch, reps)
     rows = nc - 1 if desc.link_format == _lib.LINKS_SU else nc
     bps = bytes_per_site_force(nc, D, npairs, rows)
     gbs = bps * V / (us * 1e-6) / 1e9
