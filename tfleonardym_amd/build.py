@@ -84,6 +84,26 @@ def build_cpu_check():
     return CHECK_LIB
 
 
+ASAN_BIN = os.path.join(BUILD, "asan_driver")
+
+
+def build_asan_check():
+    """Host-only AddressSanitizer build of the per-site Wilson and force
+    arithmetic (cpucheck.cpp) with a driver that runs every variant on
+    exactly sized buffers (asan_driver.cpp).  Test-only."""
+    os.makedirs(BUILD, exist_ok=True)
+    tmp = ASAN_BIN + ".tmp"
+    cmd = [HIPCC, "-O1", "-g", "-std=c++20", "-x", "hip", "--offload-host-only", f"--offload-arch={ARCH}",
+           "-Xarch_host", "-fsanitize=address", "-fno-omit-frame-pointer",
+           "-I", os.path.join(HERE, "..", "include"), os.path.join(CSRC, "cpucheck.cpp"),
+           "-x", "c++", os.path.join(CSRC, "asan_driver.cpp"), "-o", tmp]
+    rc, out = _run(cmd)
+    if rc != 0:
+        raise RuntimeError(f"asan build failed:\n{' '.join(cmd)}\n{out}")
+    os.replace(tmp, ASAN_BIN)
+    return ASAN_BIN
+
+
 def clean():
     shutil.rmtree(BUILD, ignore_errors=True)
 

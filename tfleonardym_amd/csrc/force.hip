@@ -177,12 +177,9 @@ static int launch_force(const OpDesc& d, const ForcePairs& fp, cpx<double>* out,
   if constexpr (D == 4 && !ADJ && HERM) {
     const int64_t lds = (int64_t)2 * 2 * 4 * NC * d.wthreads * 16;
     if (d.map.tiled && d.wthreads * D <= force_lds_threads<NC>() && lds <= kForceLdsMax && force_lds_enabled()) {
-      static bool attr = false;  // raise the dynamic LDS limit once per instantiation
-      if (!attr) {
-        LYM_HIP(hipFuncSetAttribute((const void*)k_force_lds<NC, HERM>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kForceLdsMax));
-        attr = true;
-      }
+      static std::atomic<uint32_t> raised;  // devices whose dynamic-LDS limit is raised
+      const int rc = raise_lds_limit((const void*)k_force_lds<NC, HERM>, (int)kForceLdsMax, raised);
+      if (rc) return rc;
       hipLaunchKernelGGL((k_force_lds<NC, HERM>), dim3(d.wblocks), dim3(d.wthreads * D), (size_t)lds, st, d.lat,
                          d.map, d.wthreads, lk, fp, vbytes, out, obytes);
       LYM_HIP(hipGetLastError());

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
 
 #include "../../include/lym.h"
@@ -64,6 +65,18 @@ int hip_fail(hipError_t e, const char* what);
     hipError_t _e = (call);                             \
     if (_e != hipSuccess) return ::lym::hip_fail(_e, #call); \
   } while (0)
+
+// Raise a kernel's dynamic-LDS limit to `bytes` on the current device, once
+// per (kernel, device): `done` is that kernel's bitmask of devices.
+inline int raise_lds_limit(const void* fn, int bytes, std::atomic<uint32_t>& done) {
+  int dev = 0;
+  LYM_HIP(hipGetDevice(&dev));
+  const uint32_t bit = 1u << (dev & 31);
+  if (done.load(std::memory_order_acquire) & bit) return LYM_OK;
+  LYM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+  done.fetch_or(bit, std::memory_order_release);
+  return LYM_OK;
+}
 
 int make_desc(const lym_wilson* w, OpDesc* d);
 int64_t field_elems(const OpDesc& d);  // complex numbers per spinor field

@@ -744,16 +744,14 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
       int hif = (P != P32 && tm.hif == 4) ? 2 : tm.hif;
       if (tm.nsite > tile_hif_threads(P)) hif = 1;
       if (lds > 64 * 1024) {
-        // above the default dynamic-LDS limit: raise it once per kernel
-        static bool raised[3] = {false, false, false};
+        // above the default dynamic-LDS limit: raise it once per kernel and device
+        static std::atomic<uint32_t> raised[3];
         const int hi = hif == 4 ? 2 : hif - 1;
-        if (!raised[hi]) {
-          const void* fn = hif == 4   ? (const void*)k_wilson_tile<NC, ADJ, P, Epi, (P == P32 ? 4 : 1)>
-                           : hif == 2 ? (const void*)k_wilson_tile<NC, ADJ, P, Epi, 2>
-                                      : (const void*)k_wilson_tile<NC, ADJ, P, Epi, 1>;
-          LYM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTileMaxLds));
-          raised[hi] = true;
-        }
+        const void* fn = hif == 4   ? (const void*)k_wilson_tile<NC, ADJ, P, Epi, (P == P32 ? 4 : 1)>
+                         : hif == 2 ? (const void*)k_wilson_tile<NC, ADJ, P, Epi, 2>
+                                    : (const void*)k_wilson_tile<NC, ADJ, P, Epi, 1>;
+        const int rc = raise_lds_limit(fn, (int)kTileMaxLds, raised[hi]);
+        if (rc) return rc;
       }
       if (hif == 4) {
         if constexpr (P == P32)
