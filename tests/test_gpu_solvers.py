@@ -138,6 +138,29 @@ def test_half_storage_cg(geometry, nc):
         assert torch.equal(row[0], single[0])
 
 
+def test_half_storage_iterations_match_complex64():
+    """16-bit r, p, t storage keeps the complex64 iteration count on point
+    sources near the critical hopping parameter: the solution since the last
+    reliable update stays complex64, so the update does not restart from a
+    residual inflated by its rounding.  Both reach 1e-10 (oracle-recomputed)."""
+    from tfleonardym_amd import dirac, dirac_solver
+    geometry = (8, 8, 8, 16)
+    print(f"case half vs complex64 iterations geometry={geometry}", flush=True)
+    op, oop = setup(geometry, 3, 0.1335, eps=0.25)
+    srcs = [fields.point_source(geometry, 4, 3, site=0, color=c, spin=s) for s, c in ((0, 0), (3, 2))]
+    its = {}
+    for mode in ("mixed", "half"):
+        s = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode=mode,
+                                           verbose=False)
+        sols = s.solve_batch(dirac.SquareOperator(op), srcs, [0.0])
+        assert s.last_status == "converged", s.last_stats
+        for b, row in zip(srcs, sols):
+            assert osv.true_residual(oop.apply_square, b, row[0].cpu().numpy(), 0.0) <= 1e-10
+        its[mode] = s.last_stats[0]["iterations"]
+    print(f"iterations {its}", flush=True)
+    assert its["half"] <= 1.15 * its["mixed"], its
+
+
 @pytest.mark.parametrize("geometry,nc", [((4, 4, 4, 4), 2), ((4, 4), 3)])
 def test_mixed_cg_adjoint(geometry, nc):
     """One shift, mixed precision, adjoint representation (Nc^2 colour

@@ -46,6 +46,16 @@ __host__ __device__ __forceinline__ cpx<R>& operator-=(cpx<R>& a, cpx<R> b) {
   a.im -= b.im;
   return a;
 }
+// a * b + c, one rounding (v_fma_f32 / v_fma_f64 on the device).  The
+// accumulating complex products below chain two of these per real part
+// (acc = fma(-z, w, fma(x, y, acc))): 4 instructions per complex
+// multiply-add, where "acc += x*y - z*w" costs 6 (the sum of the products
+// cannot be reassociated into the accumulator).
+template <typename R>
+__host__ __device__ __forceinline__ R fmad(R a, R b, R c) {
+  if constexpr (sizeof(R) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
+}
 // a * b
 template <typename R>
 __host__ __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
@@ -64,20 +74,20 @@ __host__ __device__ __forceinline__ cpx<R> cmul_c(cpx<R> a, cpx<R> b) {
 // acc += a * b
 template <typename R>
 __host__ __device__ __forceinline__ void cfma(cpx<R>& acc, cpx<R> a, cpx<R> b) {
-  acc.re += a.re * b.re - a.im * b.im;
-  acc.im += a.re * b.im + a.im * b.re;
+  acc.re = fmad(-a.im, b.im, fmad(a.re, b.re, acc.re));
+  acc.im = fmad(a.im, b.re, fmad(a.re, b.im, acc.im));
 }
 // acc += conj(a) * b
 template <typename R>
 __host__ __device__ __forceinline__ void cfmac(cpx<R>& acc, cpx<R> a, cpx<R> b) {
-  acc.re += a.re * b.re + a.im * b.im;
-  acc.im += a.re * b.im - a.im * b.re;
+  acc.re = fmad(a.im, b.im, fmad(a.re, b.re, acc.re));
+  acc.im = fmad(-a.im, b.re, fmad(a.re, b.im, acc.im));
 }
 // acc += a * conj(b)
 template <typename R>
 __host__ __device__ __forceinline__ void cfma_c(cpx<R>& acc, cpx<R> a, cpx<R> b) {
-  acc.re += a.re * b.re + a.im * b.im;
-  acc.im += a.im * b.re - a.re * b.im;
+  acc.re = fmad(a.im, b.im, fmad(a.re, b.re, acc.re));
+  acc.im = fmad(-a.re, b.im, fmad(a.im, b.re, acc.im));
 }
 template <typename R>
 __host__ __device__ __forceinline__ cpx<R> cscale(R s, cpx<R> a) {
@@ -232,6 +242,29 @@ struct SpinorView {
   __host__ __device__ __forceinline__ void load_hop(int, int, int64_t s, cpx<R> (&v)[NCOMP]) const {
     load<NCOMP>(s, v);
   }
+  // 16-bit storage: the hop kernels take a neighbour's integers as floats and
+  // its norm apart (value = v * n) and apply n once to the transported
+  // half-spinor, inside the accumulating FMAs (Wilson hops are linear in v)
+  static constexpr bool kScaled = (P == P16);
+  template <int NCOMP>
+  __host__ __device__ __forceinline__ void load_raw(int64_t s, cpx<R> (&v)[NCOMP], R& n) const {
+    if constexpr (P == P16) {
+      const uint32_t bytes = (uint32_t)(NCOMP * V * (int64_t)sizeof(E));
+      n = ld<float>(norm, (uint32_t)(V * 4), 0, V, s);
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        const E e = ld<E>(data, bytes, c, V, s);
+        v[c] = mk<R>((float)e.x, (float)e.y);
+      }
+    } else {
+      n = R(1);
+      load<NCOMP>(s, v);
+    }
+  }
+  template <int NCOMP>
+  __host__ __device__ __forceinline__ void load_hop_scaled(int, int, int64_t s, cpx<R> (&v)[NCOMP], R& n) const {
+    load_raw<NCOMP>(s, v, n);
+  }
   template <int NCOMP>
   __host__ __device__ __forceinline__ void store(int64_t s, const cpx<R> (&v)[NCOMP]) const {
     const uint32_t bytes = (uint32_t)(NCOMP * V * (int64_t)sizeof(E));
@@ -284,10 +317,12 @@ struct LinkView {
       if constexpr (NC == 3) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
+          // u[2][j] = conj(u0a u1b - u0b u1a)
           const int a = (j + 1) % 3, b = (j + 2) % 3;
-          const cpx<R> p = cmul(u[0][a], u[1][b]);
-          const cpx<R> q = cmul(u[0][b], u[1][a]);
-          u[2][j] = mk<R>(p.re - q.re, q.im - p.im);
+          const cpx<R> x = u[0][a], y = u[1][b], z = u[0][b], w = u[1][a];
+          const R re = fmad(x.re, y.re, fmad(-x.im, y.im, fmad(-z.re, w.re, z.im * w.im)));
+          const R im = fmad(z.re, w.im, fmad(z.im, w.re, fmad(-x.re, y.im, -(x.im * y.re))));
+          u[2][j] = mk<R>(re, im);
         }
       } else if constexpr (NC == 2) {
         u[1][0] = mk<R>(-u[0][1].re, u[0][1].im);

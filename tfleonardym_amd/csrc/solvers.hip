@@ -369,10 +369,10 @@ __global__ __launch_bounds__(kBlock) void k_fast_switch(SolverState* st, int64_t
 }
 
 // ---- mixed CG with reliable updates: site kernels (grid.y = RHS) -----------
-// A thread owns whole sites.  r, p and the solution since the last update,
-// x_lo, are stored at precision PV (complex64, or 16-bit fixed point with
-// per-site norms).  (16-bit x_lo under complex64 r, p was measured: point
-// sources on 24^3x48 then need 2.8x the iterations.)
+// A thread owns whole sites.  r, p and t = Q p are stored at precision PV
+// (complex64, or 16-bit fixed point with per-site norms); the solution since
+// the last update, x_lo, is always complex64 (16-bit x_lo under complex64
+// r, p was measured: point sources on 24^3x48 then need 2.8x the iterations).
 // Fields of RHS i start at data + i * NCOMP * V (and norm + i * V).
 #define LYM_SITE_STRIDE(s, V) \
   for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < (V); s += (int64_t)gridDim.x * kBlock)
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_rel_init(int64_t V, const Field* bs,
   const int rhs = blockIdx.y;
   const SpinorView<P64> b{(double2*)bs[rhs].data, nullptr, V};
   const SpinorView<PV> rv = vec_view<PV, NCOMP>(r, V, rhs), pv = vec_view<PV, NCOMP>(p, V, rhs);
-  const SpinorView<PV> xv = vec_view<PV, NCOMP>(x, V, rhs);
+  const SpinorView<P32> xv = vec_view<P32, NCOMP>(x, V, rhs);
   const SpinorView<P64> xh{x64 + (int64_t)rhs * NCOMP * V, nullptr, V};
   partials += (int64_t)rhs * gridDim.x;
   double red[1] = {0.0};
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_rel_kp(const SolverState* st, int64_
   st += rhs;
   if (st->phase != PH_KP) return;
   const SpinorView<PV> rv = vec_view<PV, NCOMP>(r, V, rhs), pv = vec_view<PV, NCOMP>(p, V, rhs);
-  const SpinorView<PV> xv = vec_view<PV, NCOMP>(x, V, rhs);
+  const SpinorView<P32> xv = vec_view<P32, NCOMP>(x, V, rhs);
   const float b = (float)st->beta_old, al = (float)st->alpha;
   const float ar = (float)st->kp_a_re, ai = (float)st->kp_a_im;
   const bool pend = st->x_pending != 0;
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_ru_add(const SolverState* st, int64_
   const int rhs = blockIdx.y;
   st += rhs;
   if (st->phase != PH_RU) return;
-  const SpinorView<PV> xv = vec_view<PV, NCOMP>(x, V, rhs);
+  const SpinorView<P32> xv = vec_view<P32, NCOMP>(x, V, rhs);
   const SpinorView<PV> pv = vec_view<PV, NCOMP>(p, V, rhs);
   const SpinorView<P64> xh{x64 + (int64_t)rhs * NCOMP * V, nullptr, V};
   const double al = st->x_pending ? st->alpha : 0.0;
@@ -477,7 +477,7 @@ template <int PV, int NCOMP>
 __global__ __launch_bounds__(kBlock) void k_rel_out(int64_t V, Field x, const double2* __restrict__ x64,
                                                     const Field* xs) {
   const int rhs = blockIdx.y;
-  const SpinorView<PV> xv = vec_view<PV, NCOMP>(x, V, rhs);
+  const SpinorView<P32> xv = vec_view<P32, NCOMP>(x, V, rhs);
   const SpinorView<P64> xh{const_cast<double2*>(x64) + (int64_t)rhs * NCOMP * V, nullptr, V};
   const SpinorView<P64> out{(double2*)xs[rhs].data, nullptr, V};
   LYM_SITE_STRIDE(s, V) {
@@ -608,7 +608,8 @@ struct Parts {
   Field tb64, tb32;         // t = Q p
   Field ps64;               // nrhs x nshift FP64 shift directions
   // mixed CG with reliable updates (one shift), every field nrhs x batched:
-  // residual, direction, Q p, solution since the last update (complex64 or 16-bit storage)
+  // residual, direction, Q p (complex64 or 16-bit storage), solution since
+  // the last update (complex64)
   Field rl_r, rl_p, rl_t, rl_x;
   Field rl_x64, rl_t64;          // complex128 accumulated solution, Q x
 };
@@ -651,7 +652,10 @@ static size_t carve(const OpDesc& d, int prec, int nshift, int nrhs, char* base,
       P.rl_r = take_batch(pl);
       P.rl_p = take_batch(pl);
       P.rl_t = take_batch(pl);
-      P.rl_x = take_batch(pl);
+      // x_lo stays complex64 under 16-bit r, p, t: a 16-bit x_lo perturbs
+      // the solution by ~3e-5 relative per site, which the next reliable
+      // update sees as a residual of ~3e-5 cond(A) and the solve loses ground
+      P.rl_x = take_batch(P32);
       P.rl_x64 = take_batch(P64);
       P.rl_t64 = take_batch(P64);
     } else {

@@ -472,6 +472,32 @@ struct TileView {
       g.template load<NCOMP>(nb, v);
     }
   }
+  // 16-bit storage: a neighbour outside the tile comes as integers and its
+  // norm (value = v * n); inside, the decoded LDS value with n = 1
+  static constexpr bool kScaled = SpinorView<P>::kScaled;
+  template <int NCOMP>
+  __device__ __forceinline__ void load_hop_scaled(int mu, int dir, int64_t nb, cpx<R> (&v)[NCOMP], R& n) const {
+    int tc = t[mu] + (dir == 0 ? 1 : -1);
+    bool inside = true;
+    if (tc < 0) {
+      tc += a[mu];
+      inside = full[mu];
+    } else if (tc >= a[mu]) {
+      tc -= a[mu];
+      inside = full[mu];
+    }
+    if (inside) {
+      const int idx = self + (tc - t[mu]) * str[mu];
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        const E e = lds[c * nsite + idx];
+        v[c] = mk<R>(e.x, e.y);
+      }
+      n = R(1);
+    } else {
+      g.template load_raw<NCOMP>(nb, v, n);
+    }
+  }
 };
 
 template <int NC, bool ADJ, int P, class Epi, int HIF>

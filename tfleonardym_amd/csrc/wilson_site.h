@@ -136,11 +136,14 @@ __host__ __device__ __forceinline__ void transport(const cpx<R> (&u)[NC][NC], co
 // One hop term (mu, dir: 0 = +mu, 1 = -mu) of a site with coordinate xmu
 // along mu, given the neighbour spinor v and the hop's link(s) u, ur
 // (forward: U_mu(x); backward: U_mu(x - mu)), accumulated into acc.
-template <int D, int NC, bool ADJ, bool PROJ, typename R>
+// SC: v is the neighbour scaled by 1/sc (16-bit storage), and sc enters the
+// accumulating FMAs (acc += sc * transported), the hop being linear in v.
+template <int D, int NC, bool ADJ, bool PROJ, typename R, bool SC = false>
 __host__ __device__ __forceinline__ void wilson_hop_apply(const Lat& lat, int mu, int dir, int xmu,
                                                           const cpx<R> (&v)[Gam<D>::NS * (ADJ ? NC * NC : NC)],
                                                           const cpx<R> (&u)[NC][NC], const cpx<R> (&ur)[NC][NC],
-                                                          cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
+                                                          cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)],
+                                                          R sc = R(1)) {
   constexpr int NS = Gam<D>::NS;
   constexpr int H = NS / 2;
   constexpr int NCC = ADJ ? NC * NC : NC;
@@ -170,10 +173,18 @@ __host__ __device__ __forceinline__ void wilson_hop_apply(const Lat& lat, int mu
       }
 #pragma unroll
       for (int c = 0; c < NCC; ++c) {
-        acc[r * NCC + c] += chi[c];
         const cpx<R> w = ipow_rt<R>(4 - kr, chi[c]);
-        acc[(H + pr) * NCC + c].re += sg * w.re;
-        acc[(H + pr) * NCC + c].im += sg * w.im;
+        if constexpr (SC) {
+          const R ssg = sg * sc;
+          acc[r * NCC + c].re = fmad(sc, chi[c].re, acc[r * NCC + c].re);
+          acc[r * NCC + c].im = fmad(sc, chi[c].im, acc[r * NCC + c].im);
+          acc[(H + pr) * NCC + c].re = fmad(ssg, w.re, acc[(H + pr) * NCC + c].re);
+          acc[(H + pr) * NCC + c].im = fmad(ssg, w.im, acc[(H + pr) * NCC + c].im);
+        } else {
+          acc[r * NCC + c] += chi[c];
+          acc[(H + pr) * NCC + c].re += sg * w.re;
+          acc[(H + pr) * NCC + c].im += sg * w.im;
+        }
       }
     }
   } else {
@@ -203,7 +214,14 @@ __host__ __device__ __forceinline__ void wilson_hop_apply(const Lat& lat, int mu
         for (int c = 0; c < NCC; ++c) chi[c] = cmul(bcf, chi[c]);
       }
 #pragma unroll
-      for (int c = 0; c < NCC; ++c) acc[sp * NCC + c] += chi[c];
+      for (int c = 0; c < NCC; ++c) {
+        if constexpr (SC) {
+          acc[sp * NCC + c].re = fmad(sc, chi[c].re, acc[sp * NCC + c].re);
+          acc[sp * NCC + c].im = fmad(sc, chi[c].im, acc[sp * NCC + c].im);
+        } else {
+          acc[sp * NCC + c] += chi[c];
+        }
+      }
     }
   }
 }
@@ -217,12 +235,14 @@ __host__ __device__ __forceinline__ void wilson_hop_one(const Lat& lat, int mu, 
                                                         cpx<R> (&acc)[Gam<D>::NS * (ADJ ? NC * NC : NC)]) {
   constexpr int NCOMP = Gam<D>::NS * (ADJ ? NC * NC : NC);
   cpx<R> v[NCOMP];
-  psi.template load_hop<NCOMP>(mu, dir, nb, v);
+  R sc = R(1);
+  if constexpr (SV::kScaled) psi.template load_hop_scaled<NCOMP>(mu, dir, nb, v, sc);
+  else psi.template load_hop<NCOMP>(mu, dir, nb, v);
   cpx<R> u[NC][NC];
   links.template load<NC>(mu, ls, u);
   cpx<R> ur[NC][NC];
   if constexpr (ADJ) links_r.template load<NC>(mu, ls, ur);
-  wilson_hop_apply<D, NC, ADJ, PROJ, R>(lat, mu, dir, xmu, v, u, ur, acc);
+  wilson_hop_apply<D, NC, ADJ, PROJ, R, SV::kScaled>(lat, mu, dir, xmu, v, u, ur, acc, sc);
 }
 
 // Hops in the order (mu, +), (mu, -), mu = 0..D-1, in groups of HIF: the
