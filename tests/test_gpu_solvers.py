@@ -119,7 +119,9 @@ def test_mixed_cg_reliable_updates(shift, mr, monkeypatch):
 def test_half_storage_cg(geometry, nc):
     """One shift with half-precision Krylov storage (per-site norms) and FP64
     reliable updates: every source reaches 1e-10 (oracle-recomputed) and
-    matches the FP64 oracle CG; batched and single solves give the same bits."""
+    matches the FP64 oracle CG; batched and single solves agree to 1e-8 (the
+    batched launches may run on their own tuned tile, whose block sums round
+    differently)."""
     from tfleonardym_amd import dirac, dirac_solver
     print(f"case half CG geometry={geometry} nc={nc}", flush=True)
     op, oop = setup(geometry, nc, 0.13)
@@ -135,7 +137,7 @@ def test_half_storage_cg(geometry, nc):
         xr, _ = osv.cg(oop.apply_square, b, 0.01, tol=1e-12)
         assert rel(xn, xr) < 1e-8
         single = s.solve(dirac.SquareOperator(op), b, [0.01])
-        assert torch.equal(row[0], single[0])
+        assert rel(xn, single[0].cpu().numpy()) < 1e-8
 
 
 def test_half_storage_iterations_match_complex64():

@@ -53,6 +53,13 @@ static std::vector<TileMap> admissible_tiles(const lym_wilson* w, int adj, int64
   return out;
 }
 
+std::vector<TileMap> tile_candidates(const OpDesc& d, int64_t lds_per_site, int max_threads, int64_t max_lds) {
+  lym_wilson w{};
+  w.ndim = d.ndim;
+  for (int mu = 0; mu < d.ndim; ++mu) w.dims[mu] = d.lat.L[mu];
+  return admissible_tiles(&w, d.adj, lds_per_site, max_threads, max_lds);
+}
+
 // Tiles chosen by lym_tune_wilson, per (lattice, Nc, representation,
 // precision index), for the whole process.
 struct TuneKey {

@@ -5,6 +5,7 @@
 
 #include <atomic>
 #include <string>
+#include <vector>
 
 #include "../../include/lym.h"
 #include "lym_reduce.h"
@@ -108,6 +109,9 @@ constexpr int kMrRhs = 3;
 inline bool wilson_mr(const OpDesc& d, int prec, int nrhs) {
   return d.use_tile && d.use_mr && prec == P32 && nrhs > 1 && d.tile_mr.on;
 }
+// every LDS tile the lattice of d admits (4D fundamental, L0 a multiple of
+// the XCD count), hop grouping 1
+std::vector<TileMap> tile_candidates(const OpDesc& d, int64_t lds_per_site, int max_threads, int64_t max_lds);
 inline int wilson_blocks(const OpDesc& d, int prec, int nrhs = 1) {
   if (wilson_mr(d, prec, nrhs)) return d.tile_mr.blocks;
   return wilson_tiled(d, prec) ? d.tile[prec == P64 ? 0 : 1].blocks : d.wblocks;

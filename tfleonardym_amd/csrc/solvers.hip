@@ -12,7 +12,10 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -1106,6 +1109,98 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
 //   [x_hi += x_lo + alpha p ; t = Q x_hi ; r = b - (Q t + sigma x_hi) (|r|^2, <r,p>) ; test]
 constexpr int kRelChunk = 8;  // iterations per host chunk
 
+// Tile of the batched Wilson kernels of the reliable-update CG.  The shape
+// that is fastest for one RHS is not the fastest for a batch (measured at
+// 24^3x48 with 28 RHS: a 192-site tile with two hops in flight beats the
+// one-RHS choice by 5%), so the first batched solve on a lattice times its
+// step-1 kernel (t = Q p with |t|^2, |p|^2, on the solve's own fields, right
+// after the initialisation) over every candidate tile and hop grouping, and
+// the choice holds for the process, per lattice, Nc, storage precision and
+// batch size.  LYM_NO_BATCH_TUNE=1 keeps the one-RHS tile.
+struct BatchTuneKey {
+  int nc, pl, nrhs, L[4];
+  bool operator<(const BatchTuneKey& o) const {
+    return std::tie(nc, pl, nrhs, L[0], L[1], L[2], L[3]) < std::tie(o.nc, o.pl, o.nrhs, o.L[0], o.L[1], o.L[2], o.L[3]);
+  }
+};
+static std::mutex g_btune_mu;
+static std::map<BatchTuneKey, TileMap> g_btuned;
+
+static bool batch_tune_enabled() {
+  const char* v = std::getenv("LYM_NO_BATCH_TUNE");
+  return !(v && v[0] == '1');
+}
+
+// db = d with the batched tile in d.tile[1] (unchanged when the lattice is
+// not tiled, tuning is off, or no RHS is active).  Leaves P.rl_t and the
+// partials overwritten; the states are only read.
+static int batched_tile(const OpDesc& d, const Parts& P, int pl, int nrhs, hipStream_t st, OpDesc& db) {
+  db = d;
+  if (nrhs < 2 || !wilson_tiled(d, pl) || d.adj || d.ndim != 4 || !batch_tune_enabled()) return LYM_OK;
+  const BatchTuneKey key{d.nc, pl, nrhs, {d.lat.L[0], d.lat.L[1], d.lat.L[2], d.lat.L[3]}};
+  {
+    std::lock_guard<std::mutex> lock(g_btune_mu);
+    auto it = g_btuned.find(key);
+    if (it != g_btuned.end()) {
+      db.tile[1] = it->second;
+      return LYM_OK;
+    }
+  }
+  BatchCtl c{};
+  LYM_HIP(hipMemcpyAsync(&c, P.ctl, sizeof(c), hipMemcpyDeviceToHost, st));
+  LYM_HIP(hipStreamSynchronize(st));
+  if (c.ndone >= nrhs) return LYM_OK;  // every source is zero: nothing to time
+  std::vector<TileMap> cands;
+  for (const TileMap& t : tile_candidates(d, (int64_t)d.ncomp * 8, kTileMaxThreads, kTileMaxLds)) {
+    for (int hif : {1, 2}) {
+      if (hif > 1 && t.nsite > tile_hif_threads(pl)) continue;
+      TileMap x = t;
+      x.hif = hif;
+      cands.push_back(x);
+    }
+  }
+  if (cands.empty()) return LYM_OK;
+  hipEvent_t e0, e1;
+  LYM_HIP(hipEventCreate(&e0));
+  LYM_HIP(hipEventCreate(&e1));
+  EpiArgs a1;
+  a1.out = P.rl_t;
+  a1.st = P.st;
+  a1.rhs_stride = field_elems(d);
+  int rc = LYM_OK;
+  double best_us = 1e30;
+  TileMap best = d.tile[1];
+  const int reps = 3;
+  for (size_t ci = 0; ci < cands.size() && !rc; ++ci) {
+    OpDesc dc = d;
+    dc.tile[1] = cands[ci];
+    for (int i = 0; i < 2 && !rc; ++i) rc = wilson_launch(EPI_CGM_K1, dc, pl, true, P.rl_p, a1, redctl(P), st, nrhs);
+    double us = 1e30;
+    for (int trial = 0; trial < 2 && !rc; ++trial) {
+      if (hipEventRecord(e0, st) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventRecord");
+      for (int i = 0; i < reps && !rc; ++i) rc = wilson_launch(EPI_CGM_K1, dc, pl, true, P.rl_p, a1, redctl(P), st, nrhs);
+      if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventRecord");
+      float ms = 0.f;
+      if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventSynchronize");
+      if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventElapsedTime");
+      if (!rc) us = std::min(us, 1e3 * ms / reps);
+    }
+    if (!rc && us < best_us) {
+      best_us = us;
+      best = cands[ci];
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lock(g_btune_mu);
+    g_btuned[key] = best;
+  }
+  db.tile[1] = best;
+  return LYM_OK;
+}
+
 static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* const* bs, void* const* xs, int nrhs,
                        double sigma, double tol, int max_iter, hipStream_t st, std::vector<SolverState>& out_states) {
   graph_new_solve();
@@ -1157,7 +1252,9 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   if (rc) return rc;
   rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
   if (rc) return rc;
-  const int gwl = wilson_blocks(d, pl, nrhs), gw64 = wilson_blocks(d, P64, nrhs);
+  OpDesc db;
+  if ((rc = batched_tile(d, P, pl, nrhs, st, db))) return rc;
+  const int gwl = wilson_blocks(db, pl, nrhs), gw64 = wilson_blocks(db, P64, nrhs);
   // the update accumulation is mostly gated off: a small grid-stride grid
   const int gsr = std::min(gs, 1024);
   static_assert(kRelChunk % 2 == 0, "the last iteration of a chunk must carry the update kernels");
@@ -1173,7 +1270,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
       a1.out = P.rl_t;
       a1.st = P.st;
       a1.rhs_stride = n;
-      e = wilson_launch(EPI_CGM_K1, d, pl, true, P.rl_p, a1, redctl(P), st, nrhs);
+      e = wilson_launch(EPI_CGM_K1, db, pl, true, P.rl_p, a1, redctl(P), st, nrhs);
       if (!e) e = launch_fin(FK_CG_A, P.st, P.partials, gwl, 2, nullptr, st, nrhs, P.ctl);
       if (e) return e;
       EpiArgs a2;
@@ -1181,7 +1278,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
       a2.aux1 = P.rl_r;
       a2.st = P.st;
       a2.rhs_stride = n;
-      e = wilson_launch(EPI_CG_K2X, d, pl, true, P.rl_t, a2, redctl(P), st, nrhs);
+      e = wilson_launch(EPI_CG_K2X, db, pl, true, P.rl_t, a2, redctl(P), st, nrhs);
       if (!e) e = launch_fin(FK_CG_B, P.st, P.partials, gwl, 1, nullptr, st, nrhs, P.ctl);
       if (e) return e;
       // reliable update after every second iteration (an RHS that asks for
@@ -1197,7 +1294,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
       a3.out = P.rl_t64;
       a3.st = P.st;
       a3.rhs_stride = n;
-      e = wilson_launch(EPI_RU1, d, P64, true, P.rl_x64, a3, RedCtl{nullptr, nullptr}, st, nrhs);
+      e = wilson_launch(EPI_RU1, db, P64, true, P.rl_x64, a3, RedCtl{nullptr, nullptr}, st, nrhs);
       if (e) return e;
       EpiArgs a4;
       a4.out = P.rl_r;
@@ -1206,7 +1303,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
       a4.bs = P.bs_dev;
       a4.st = P.st;
       a4.rhs_stride = n;
-      e = wilson_launch(pl == P16 ? EPI_RU2H : EPI_RU2, d, P64, true, P.rl_t64, a4, redctl(P), st, nrhs);
+      e = wilson_launch(pl == P16 ? EPI_RU2H : EPI_RU2, db, P64, true, P.rl_t64, a4, redctl(P), st, nrhs);
       if (!e) e = launch_fin(FK_CG_RU, P.st, P.partials, gw64, 3, nullptr, st, nrhs, P.ctl);
       if (e) return e;
     }
@@ -1219,7 +1316,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   int64_t issued = 0, polled = 0;
   KeyBuf kb;
   kb.add('R').add(pl).add(nrhs).add(P.st).add(P.xs_dev).add(P.rl_r.data);
-  const std::string key = op_key(d) + kb.s;
+  const std::string key = op_key(db) + kb.s;
   auto issue = [&](int slot) -> int {
     const int e = launch_chunk(key, st, chunk);
     if (e) return e;
