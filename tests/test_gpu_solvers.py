@@ -163,6 +163,32 @@ def test_half_storage_iterations_match_complex64():
     assert its["half"] <= 1.15 * its["mixed"], its
 
 
+@pytest.mark.parametrize("mode", ["half", "mixed"])
+def test_batch_tuned_tile(mode, monkeypatch):
+    """The batched reliable-update CG on its own tuned tile (the first batched
+    solve of a lattice and batch size times the candidates) and on the one-RHS
+    tile (LYM_NO_BATCH_TUNE=1): both reach 1e-10 (oracle-recomputed) and agree
+    to 1e-8; a second tuned solve reuses the recorded tile bit for bit."""
+    from tfleonardym_amd import dirac, dirac_solver
+    geometry = (8, 8, 4, 16)
+    print(f"case batch-tuned tile mode={mode}", flush=True)
+    op, oop = setup(geometry, 3, 0.13)
+    srcs = [fields.gaussian_spinor(geometry, 4, 3, seed=90 + i) for i in range(3)]
+    runs = {}
+    for tag, env in (("tuned", "0"), ("again", "0"), ("fixed", "1")):
+        monkeypatch.setenv("LYM_NO_BATCH_TUNE", env)
+        s = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode=mode,
+                                           verbose=False)
+        sols = s.solve_batch(dirac.SquareOperator(op), srcs, [0.0])
+        assert s.last_status == "converged", (tag, s.last_stats)
+        runs[tag] = [row[0] for row in sols]
+    for b, xt, xa, xf in zip(srcs, runs["tuned"], runs["again"], runs["fixed"]):
+        assert torch.equal(xt, xa)
+        for x in (xt, xf):
+            assert osv.true_residual(oop.apply_square, b, x.cpu().numpy(), 0.0) <= 1e-10
+        assert rel(xt, xf.cpu().numpy()) < 1e-8
+
+
 @pytest.mark.parametrize("geometry,nc", [((4, 4, 4, 4), 2), ((4, 4), 3)])
 def test_mixed_cg_adjoint(geometry, nc):
     """One shift, mixed precision, adjoint representation (Nc^2 colour
