@@ -1132,11 +1132,13 @@ static bool batch_tune_enabled() {
 }
 
 // db = d with the batched tile in d.tile[1] (unchanged when the lattice is
-// not tiled, tuning is off, or no RHS is active).  Leaves P.rl_t and the
+// not tiled, the multi-RHS kernel runs the batch, tuning is off, or no RHS
+// is active).  Leaves P.rl_t and the
 // partials overwritten; the states are only read.
 static int batched_tile(const OpDesc& d, const Parts& P, int pl, int nrhs, hipStream_t st, OpDesc& db) {
   db = d;
-  if (nrhs < 2 || !wilson_tiled(d, pl) || d.adj || d.ndim != 4 || !batch_tune_enabled()) return LYM_OK;
+  if (nrhs < 2 || !wilson_tiled(d, pl) || wilson_mr(d, pl, nrhs) || d.adj || d.ndim != 4 || !batch_tune_enabled())
+    return LYM_OK;
   const BatchTuneKey key{d.nc, pl, nrhs, {d.lat.L[0], d.lat.L[1], d.lat.L[2], d.lat.L[3]}};
   {
     std::lock_guard<std::mutex> lock(g_btune_mu);
