@@ -251,6 +251,31 @@ def test_float32_apply():
     assert rel(out.to(torch.complex128), want) < TOL32
 
 
+@pytest.mark.parametrize("geometry", [(16, 16, 16, 32), (24, 24, 24, 48)])
+def test_float32_apply_full_size(geometry):
+    """complex64 D on the benchmark lattices (rebuilt link rows, antiperiodic
+    t) against the complex128 D on the same input: the largest component
+    difference stays within 4e-7 of the largest component, the rounding of
+    complex64 arithmetic on a 9-term sum."""
+    import ctypes
+    from tfleonardym_amd import _lib
+    print(f"case f32 vs f64 D geometry={geometry}", flush=True)
+    U = fields.gauge_field(geometry, 3, seed=20250, kind="near_identity", eps=0.25)
+    op = make_op(U, geometry, 0.1335, True, "fundamental", -1.0)
+    desc = op.ensure_f32()
+    g = torch.Generator(device="cpu").manual_seed(3)
+    x64 = torch.randn(op.spinor_shape, dtype=torch.complex128, generator=g).cuda()
+    y64 = torch.empty_like(x64)
+    _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), _lib.PREC_F64, 0, _lib.ptr(x64), _lib.ptr(y64),
+                                           _lib.stream_handle()))
+    x32 = x64.to(torch.complex64)
+    y32 = torch.empty_like(x32)
+    _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), _lib.PREC_F32, 0, _lib.ptr(x32), _lib.ptr(y32),
+                                           _lib.stream_handle()))
+    err = (y32.to(torch.complex128) - y64).abs().max().item() / y64.abs().max().item()
+    assert np.isfinite(err) and err < 4e-7, err
+
+
 def test_c2_size_parity_and_hermiticity():
     """16^3 x 32 SU(3) (the multishift workload's lattice): FP64 parity and
     <v2, Q v1> = <Q v2, v1> at full size."""
