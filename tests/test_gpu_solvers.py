@@ -273,6 +273,29 @@ def test_non_finite_source_is_a_breakdown(mode, shifts):
             assert osv.true_residual(oop.apply_square, good, xs[2][k].cpu().numpy(), sh) <= 1e-10
 
 
+@pytest.mark.parametrize("mode", ["half", "mixed"])
+def test_zero_sources_in_a_batch(mode):
+    """Zero sources inside a batch on a tiled lattice return x = 0 as
+    converged while the others converge to 1e-10; a batch of zero sources
+    only (nothing for the batch tile tuning to time) returns zeros."""
+    from tfleonardym_amd import dirac, dirac_solver
+    geometry = (8, 4, 4, 16)
+    print(f"case zero sources mode={mode}", flush=True)
+    op, oop = setup(geometry, 3, 0.13)
+    good = fields.gaussian_spinor(geometry, 4, 3, seed=19)
+    zero = np.zeros_like(good)
+    A = dirac.SquareOperator(op)
+    s = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode=mode,
+                                       verbose=False)
+    xs = s.solve_batch(A, [zero, good, zero], [0.0])
+    assert [x["status"] for x in s.last_stats] == ["converged"] * 3
+    assert not xs[0][0].abs().max().item() and not xs[2][0].abs().max().item()
+    assert osv.true_residual(oop.apply_square, good, xs[1][0].cpu().numpy(), 0.0) <= 1e-10
+    xz = s.solve_batch(A, [zero, zero], [0.0])
+    assert s.last_status == "converged"
+    assert all(not row[0].abs().max().item() for row in xz)
+
+
 def test_c1_plain_fp64_cg():
     """The CPU-runnable workload: 2D 24x16 SU(2), Haar links, kappa=-0.15,
     periodic x / antiperiodic t, FP64 CG on D^dag D to 1e-10."""
