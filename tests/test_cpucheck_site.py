@@ -75,3 +75,50 @@ def test_adjoint_custom_provider_two_link_fields(lib):
     want = ow.apply_projected_adjoint(Ul, U, X, 0.11, True, __import__("oracle").lattice.lookup_tables(geometry))
     got = run(lib, geometry, nc, 1, 1, 0.11, 1.0 + 0j, 1, Ul, U, X)
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-14
+
+
+def encode16(psi):
+    """16-bit storage of a spinor as the kernels write it: per site the norm
+    n = max |component| / 32767 (float32) and int16 = rint(value * 32767 / max)."""
+    ns, nc, V = psi.shape
+    comp = np.stack([psi.real, psi.imag], axis=-1).reshape(ns * nc, V, 2)
+    m = np.abs(comp).max(axis=(0, 2)).astype(np.float32)
+    norm = (m * np.float32(1.0 / 32767.0)).astype(np.float32)
+    inv = np.where(m > 0, np.float32(32767.0) / np.where(m > 0, m, 1), 0).astype(np.float32)
+    ints = np.clip(np.rint(comp.astype(np.float32) * inv[None, :, None]), -32767, 32767).astype(np.int16)
+    return ints, norm
+
+
+@pytest.mark.parametrize("geometry,nc", [((4, 6), 2), ((3, 4), 3), ((2, 4, 2, 4), 2), ((4, 2, 3, 4), 3)])
+@pytest.mark.parametrize("herm", [0, 1])
+@pytest.mark.parametrize("bc", [None, -1.0])
+@pytest.mark.parametrize("proj", [1, 0])
+def test_half_storage_hop_matches_oracle(lib, geometry, nc, herm, bc, proj):
+    """The 16-bit input path of the hop (neighbour integers converted to float
+    and the site norm folded into the accumulating FMAs), complex64 links and
+    arithmetic: equal to the oracle applied to the decoded input within
+    complex64 rounding."""
+    ndim = len(geometry)
+    ns = 2 if ndim == 2 else 4
+    U = fields.gauge_field(geometry, nc, seed=21, kind="haar")
+    psi = fields.gaussian_spinor(geometry, ns, nc, seed=22)
+    psi[:, :, 1] *= 1e-3  # sites of very different size keep their own scale
+    ints, norm = encode16(psi)
+    V = psi.shape[-1]
+    dec = (ints[..., 0].astype(np.float64) + 1j * ints[..., 1].astype(np.float64)) * norm[None, :].astype(np.float64)
+    dec = dec.reshape(ns, nc, V)
+    kappa = 0.1234
+    want = ow.WilsonOracle(U, geometry, kappa, hermitian=bool(herm), representation="fundamental",
+                           bc_factor=bc).apply(dec)
+    lib.lymcheck_wilson_apply_half.restype = ctypes.c_int
+    dims = (ctypes.c_int * 4)(*(list(geometry) + [1] * (4 - ndim)))
+    links = np.ascontiguousarray(U.astype(np.complex64))
+    out = np.zeros(psi.shape, dtype=np.complex64)
+    b = 1.0 + 0j if bc is None else complex(bc)
+    rc = lib.lymcheck_wilson_apply_half(ndim, nc, herm, dims, ctypes.c_double(kappa), ctypes.c_double(b.real),
+                                        ctypes.c_double(b.imag), proj, links.ctypes.data_as(ctypes.c_void_p),
+                                        np.ascontiguousarray(ints).ctypes.data_as(ctypes.c_void_p),
+                                        norm.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    err = np.linalg.norm(out.astype(np.complex128) - want) / np.linalg.norm(want)
+    assert err < 2e-6, err

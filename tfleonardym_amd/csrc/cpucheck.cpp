@@ -4,6 +4,8 @@
 // against the oracle without a GPU.  Not part of liblym.so.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "force_site.h"
 #include "wilson_site.h"
 
@@ -137,6 +139,44 @@ extern "C" int lymcheck_wilson_apply(int ndim, int nc, int adj, int herm, const 
     case 431: return run_mode<4, 3, true>(proj, lat, links, links_r, in, out);
     case 221: return run_mode<2, 2, true>(proj, lat, links, links_r, in, out);
     case 231: return run_mode<2, 3, true>(proj, lat, links, links_r, in, out);
+  }
+  return -1;
+}
+
+// 16-bit storage path of the hop (fundamental): the input as int16 pairs with
+// one float norm per site (value = int16 * norm), complex64 links and
+// arithmetic, complex64 output.  Exercises the neighbour loads that take the
+// integers and the norm apart and fold the norm into the accumulating FMAs.
+template <int D, int NC, bool PROJ>
+static int run_half(const Lat& lat, const float* links, const short* in, const float* norm, float* out) {
+  constexpr int NCOMP = Gam<D>::NS * NC;
+  SpinorView<P16> sin{(short2*)in, const_cast<float*>(norm), lat.V};
+  SpinorView<P32> sout{(float2*)out, nullptr, lat.V};
+  LinkView<float> lk{(const cpx<float>*)links, lat.V};
+  for (int64_t s = 0; s < lat.V; ++s) {
+    cpx<float> hop[NCOMP], c[NCOMP], res[NCOMP];
+    wilson_hop<D, NC, false, PROJ, float>(lat, s, sin, lk, lk, hop);
+    sin.template load<NCOMP>(s, c);
+    wilson_finish<NCOMP, float>(lat, c, hop, res);
+    sout.template store<NCOMP>(s, res);
+  }
+  return 0;
+}
+
+extern "C" int lymcheck_wilson_apply_half(int ndim, int nc, int herm, const int* dims, double kappa, double bc_re,
+                                          double bc_im, int proj, const float* links, const short* in,
+                                          const float* norm, float* out) {
+  const Lat lat = make_lat(ndim, herm, dims, kappa, bc_re, bc_im);
+  const int key = ndim * 10 + nc;
+  auto go = [&](auto d, auto n) {
+    constexpr int DD = decltype(d)::value, NN = decltype(n)::value;
+    return proj ? run_half<DD, NN, true>(lat, links, in, norm, out) : run_half<DD, NN, false>(lat, links, in, norm, out);
+  };
+  switch (key) {
+    case 42: return go(std::integral_constant<int, 4>{}, std::integral_constant<int, 2>{});
+    case 43: return go(std::integral_constant<int, 4>{}, std::integral_constant<int, 3>{});
+    case 22: return go(std::integral_constant<int, 2>{}, std::integral_constant<int, 2>{});
+    case 23: return go(std::integral_constant<int, 2>{}, std::integral_constant<int, 3>{});
   }
   return -1;
 }

@@ -183,7 +183,7 @@ inline int64_t max_red_blocks(int64_t V) {
 }
 constexpr int kMaxRed = 4;  // most values any kernel reduces
 constexpr int kTileMaxThreads = 1024;   // LDS-tiled Wilson block size limit (complex64 / 16-bit, one hop in flight)
-constexpr int kTileMaxThreads64 = 640;  // the same for complex128 (165 VGPRs: at most 10 waves per block)
+constexpr int kTileMaxThreads64 = 640;  // the same for complex128 (166 VGPRs: at most 10 waves per block)
 constexpr int kTileMaxThreadsMr = 640;  // multi-RHS tile
 // blocks that keep several hops in flight need more registers per thread:
 constexpr int kTileHifThreads32 = 512;  // complex64 / 16-bit, HIF 2 and 4
