@@ -116,15 +116,16 @@ def run_serving(args):
     print(json.dumps(out))
 
 
-def run_batched(args):
+def batched_result(workload, precision, steps, warmup, baseline_steps, autotune=True):
     """Batched multi-RHS CG (all sources in the same launches; 16-bit Krylov
-    storage with FP32 arithmetic and FP64 reliable updates unless --precision
+    storage with FP32 arithmetic and FP64 reliable updates unless `precision`
     says otherwise) against the plain FP64 solver run source by source, on
-    sources spread over both kinds (point and Z2)."""
+    sources spread over both kinds (point and Z2).  Returns the result dict
+    (None values where skipped); exits if a solve misses 1e-10."""
     import torch
     from tfleonardym_amd import _lib, dirac, dirac_solver, fields, lattice
     dev = _lib.require_device()
-    W = BATCHED[args.workload]
+    W = BATCHED[workload]
     geometry, nc = W["geometry"], W["nc"]
     U = fields.gauge_field(geometry, nc, seed=20250, kind="near_identity", eps=W["eps"])
     srcs = [fields.point_source(geometry, 4, nc, site=0, color=c, spin=s) for s in range(4) for c in range(nc)]
@@ -132,32 +133,32 @@ def run_batched(args):
     srcs = [torch.from_numpy(s).to(dev) for s in srcs]
     op = dirac.DiracWilsonOperator(lattice.Configuration(U, geometry, nc, 4), W["kappa"], True,
                                    representation="fundamental")
-    tiles = op.autotune() if not args.no_autotune else None
+    tiles = op.autotune() if autotune else {"f64": op.tile_info("f64"), "f32": op.tile_info("f32")}
     A = dirac.SquareOperator(op)
-    mode = args.precision or W["precision"]
+    mode = precision or W["precision"]
     solver = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode=mode,
                                             verbose=False)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         solver.solve_batch(A, srcs, [0.0])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(steps):
         solver.solve_batch(A, srcs, [0.0])
         st = solver.last_stats
         worst = max(max(s["true_rel_resid"]) for s in st)
-        print(f"[bench] batched step {i}: {st[0]['seconds'] * 1e3:.0f} ms, {st[0]['iterations']} it, "
+        print(f"[bench] batched {workload} step {i}: {st[0]['seconds'] * 1e3:.0f} ms, {st[0]['iterations']} it, "
               f"max true residual {worst:.2e}", file=sys.stderr, flush=True)
         if solver.last_status != "converged" or worst > 1e-10:
             print(f"[bench] batched solve did not reach 1e-10", file=sys.stderr)
             sys.exit(1)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
-    value = args.steps * len(srcs) / total
+    value = steps * len(srcs) / total
     plain = None
-    if args.baseline_steps > 0:
+    if baseline_steps > 0:
         ps = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="double",
                                             verbose=False)
-        nb = min(len(srcs), max(2, args.baseline_steps))
+        nb = min(len(srcs), max(2, baseline_steps))
         pick = sorted(set(np.linspace(0, len(srcs) - 1, nb).round().astype(int).tolist()))
         ps.solve(A, srcs[0], [0.0])
         torch.cuda.synchronize()
@@ -167,20 +168,33 @@ def run_batched(args):
         torch.cuda.synchronize()
         plain = len(pick) / (time.perf_counter() - tb)
         print(f"[bench] plain FP64 CG, one source at a time: {plain:.2f} solves/s", file=sys.stderr, flush=True)
-    out = {
+    return {
         "metric": "batched multi-RHS CG solves/s at 1e-10 true resid",
-        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": total / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None,
+        "value": value, "unit": "solves/s", "steps": steps, "warmup": warmup,
+        "ms_per_step": total / steps * 1e3,
         "dtype": {"half": "f16 storage + f32 arithmetic + f64 reliable updates", "mixed": "f32+f64",
                   "double": "f64"}[mode],
-        "data": "synthetic (seeded near-identity SU(3) links; point + Z2 sources)",
         "config": {"workload": W["name"], "geometry": list(geometry), "kappa": W["kappa"], "rhs": len(srcs),
                    "precision": mode, "tolerance": 1e-10},
         "iterations": solver.last_stats[0]["iterations"],
         "plain_fp64_single_rhs_solves_per_s": plain,
         "speedup_vs_plain_fp64": (value / plain) if plain else None,
         "wilson_tiles": tiles,
+    }
+
+
+def run_batched(args):
+    """The batched workloads as the bench's own JSON line."""
+    r = batched_result(args.workload, args.precision, args.steps, args.warmup, args.baseline_steps,
+                       autotune=not args.no_autotune)
+    out = {
+        "metric": r["metric"], "value": r["value"], "unit": r["unit"], "n_gpus": 1, "steps": r["steps"],
+        "warmup": r["warmup"], "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": r["dtype"],
+        "data": "synthetic (seeded near-identity SU(3) links; point + Z2 sources)",
+        "config": r["config"], "iterations": r["iterations"],
+        "plain_fp64_single_rhs_solves_per_s": r["plain_fp64_single_rhs_solves_per_s"],
+        "speedup_vs_plain_fp64": r["speedup_vs_plain_fp64"], "wilson_tiles": r["wilson_tiles"],
     }
     print(json.dumps(out))
 
@@ -395,6 +409,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--dslash-reps", type=int, default=50)
     ap.add_argument("--no-dslash-c3", action="store_true", help="skip the 24^3x48 dslash measurement")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched 28-source CG at 24^3x48 reported beside the default workload")
     ap.add_argument("--dslash-geometry", default="24x24x24x48", help="lattice of --workload dslash")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in Wilson tile rule")
     ap.add_argument("--baseline-steps", type=int, default=2,
@@ -487,6 +503,11 @@ def main():
     if not args.no_dslash_c3:
         res_c3 = dslash_c3(args.dslash_reps, autotune=not args.no_autotune)
 
+    # ---- the batched 28-source CG at 24^3x48 (16-bit storage) beside it ---
+    res_b = None
+    if not args.no_batched and args.workload == "c2":
+        res_b = batched_result("c3b", None, 1, 1, 2, autotune=args.no_dslash_c3 and not args.no_autotune)
+
     # ---- CPU baseline: the oracle (NumPy restatement of the reference op
     # sequence), a bounded number of FP64 CG-M iterations, extrapolated ------
     cpu = None
@@ -534,6 +555,7 @@ def main():
                      "traffic": measured_traffic(geometry, "f32"),
                      "bytes_per_site": f32["bytes_per_site"], "sites": V, "avg_us": f32["us"]},
         "dslash_24c3x48": res_c3,
+        "batched_24c3x48": res_b,
         "wilson_tiles": {"16c3x32": tiles or {"f32": op.tile_info("f32"), "f64": op.tile_info("f64")},
                          "24c3x48": res_c3.get("tiles") if res_c3 else None},
         "cpu_baseline": cpu,
