@@ -81,7 +81,7 @@ def run_serving(args):
     print(f"[bench] serving: {nreq} requests over {t:.3f} s of arrivals, {len(keys)} resident fields "
           f"({store.resident_bytes() / 2**30:.2f} GiB)", file=sys.stderr, flush=True)
     results = []
-    with SolveServer(store, slots=W["slots"]) as srv:
+    with SolveServer(store, slots=W["slots"], precision_mode=args.precision) as srv:
         # warm the kernels once per field
         for k, g, nc in keys:
             srv.submit(SolveRequest(field=k, kappa=0.12, shifts=[1.0],
@@ -108,8 +108,11 @@ def run_serving(args):
         "metric": "serving solves/s at 1e-10 true resid, p50/p99 latency",
         "value": nreq / (t1 - t0), "unit": "solves/s", "n_gpus": 1, "steps": nreq, "warmup": len(keys),
         "ms_per_step": (t1 - t0) / nreq * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64", "data": "synthetic (seeded Poisson arrivals, near-identity SU(2)/SU(3) fields, Gaussian sources)",
-        "config": {"workload": W["name"], "rate_per_s": W["rate"], "slots": W["slots"], "fields": len(keys)},
+        "dtype": {"mixed": "f32+f64", "double": "f64", "half": "f16 storage (one-shift requests) + f32 + f64"}[
+            args.precision],
+        "data": "synthetic (seeded Poisson arrivals, near-identity SU(2)/SU(3) fields, Gaussian sources)",
+        "config": {"workload": W["name"], "rate_per_s": W["rate"], "slots": W["slots"], "fields": len(keys),
+                   "precision": args.precision},
         "latency_ms": {"p50": lat["p50_ms"], "p99": lat["p99_ms"], "mean": lat["mean_ms"]},
         "max_inflight": inflight,
     }

@@ -39,22 +39,25 @@ def make_requests(links, n=10, seed=5):
     return reqs
 
 
-def run(store, reqs, slots):
+def run(store, reqs, slots, precision_mode="mixed"):
     from tfleonardym_amd.serving import SolveRequest, SolveServer
-    with SolveServer(store, slots=slots) as srv:
+    with SolveServer(store, slots=slots, precision_mode=precision_mode) as srv:
         futs = [srv.submit(SolveRequest(r.field, r.kappa, r.shifts, r.source, rid=r.rid)) for r in reqs]
         out = {f.result(timeout=120).rid: f.result() for f in futs}
         inflight = srv.max_inflight_seen
     return out, inflight
 
 
-def test_mix_equals_alone_and_meets_tolerance():
+@pytest.mark.parametrize("mode", ["mixed", "half"])
+def test_mix_equals_alone_and_meets_tolerance(mode):
+    """mode "half": the one-shift requests run with 16-bit Krylov storage."""
+    print(f"case serving mode={mode}", flush=True)
     store, links = build_store()
     reqs = make_requests(links)
-    mixed, inflight = run(store, reqs, slots=4)
+    mixed, inflight = run(store, reqs, slots=4, precision_mode=mode)
     alone = {}
     for r in reqs:
-        res, _ = run(store, [r], slots=1)
+        res, _ = run(store, [r], slots=1, precision_mode=mode)
         alone.update(res)
     assert inflight >= 2  # requests did overlap
     for r in reqs:

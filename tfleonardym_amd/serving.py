@@ -120,7 +120,10 @@ class GaugeStore:
 class SolveServer:
     """Continuous-admission multishift solve server (see module docstring)."""
 
-    _PREC = {"mixed": _lib.PREC_F32, "double": _lib.PREC_F64}
+    # "half": one-shift requests run the 16-bit-storage CG with FP64 reliable
+    # updates, multishift requests the mixed CG-M (16-bit storage is built
+    # for one shift only)
+    _PREC = {"mixed": _lib.PREC_F32, "double": _lib.PREC_F64, "half": _lib.PREC_F16}
 
     def __init__(self, store, slots=4, hbm_budget_bytes=None, precision_mode="mixed", max_iter=50000,
                  max_wait_s=2.0):
@@ -165,11 +168,15 @@ class SolveServer:
         self.stop()
 
     # ---- admission -----------------------------------------------------------
+    def _prec_for(self, nshift):
+        """Storage precision of a request with `nshift` shifts."""
+        return self.prec if (self.prec != _lib.PREC_F16 or nshift == 1) else _lib.PREC_F32
+
     def footprint(self, req):
         """Device bytes a request holds while it runs."""
         d = self.store.desc(req.field, req.kappa)
         n = len(req.shifts)
-        ws = int(_lib.lib().lym_solver_workspace_bytes(ctypes.byref(d), self.prec, n))
+        ws = int(_lib.lib().lym_solver_workspace_bytes(ctypes.byref(d), self._prec_for(n), n))
         if ws == 0:
             raise RuntimeError(_lib.lib().lym_last_error().decode())
         elems = int(np.prod(self.store.spinor_shape(req.field)))
@@ -242,13 +249,14 @@ class SolveServer:
         src = to_device(req.source, device=dev)
         if tuple(src.shape) != self.store.spinor_shape(req.field):
             raise RuntimeError(f"source shape {tuple(src.shape)} does not match field {req.field}")
-        nbytes = int(lib.lym_solver_workspace_bytes(ctypes.byref(d), self.prec, n))
+        prec = self._prec_for(n)
+        nbytes = int(lib.lym_solver_workspace_bytes(ctypes.byref(d), prec, n))
         ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=dev)
         xs = [torch.empty_like(src) for _ in shifts]
         sh = (ctypes.c_double * n)(*shifts)
         xp = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
         st = _lib.LymSolveStats()
-        rc = _lib.check(lib.lym_cgm(ctypes.byref(d), self.prec, n, sh, _lib.ptr(src), xp, float(req.tolerance),
+        rc = _lib.check(lib.lym_cgm(ctypes.byref(d), prec, n, sh, _lib.ptr(src), xp, float(req.tolerance),
                                     self.max_iter, _lib.ptr(ws), ws.numel(), ctypes.byref(st),
                                     ctypes.c_void_p(stream.cuda_stream)), "serve")
         stream.synchronize()
