@@ -550,6 +550,7 @@ def main():
         "plain_fp64_solves_per_s": plain,
         "speedup_vs_plain_fp64": (solves_per_s / plain) if plain else None,
         "max_true_rel_resid": max(stats["true_rel_resid"]),
+        "true_rel_resid_per_shift": [float(v) for v in stats["true_rel_resid"]],
         "dslash_gbs": {"f64": res["f64"]["gbs"], "f32": f32["gbs"]},
         "roofline": {"kernel": "Wilson D complex64, one RHS (k_wilson_tile)", "bound": "hbm",
                      "achieved": f32["gbs"], "link_format": "su-rebuilt" if desc.link_format == _lib.LINKS_SU
