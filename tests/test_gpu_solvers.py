@@ -296,6 +296,29 @@ def test_zero_sources_in_a_batch(mode):
     assert all(not row[0].abs().max().item() for row in xz)
 
 
+def test_batched_half_cg_full_size():
+    """The batched 16-bit CG at 24^3x48 SU(3), kappa = 0.1335 (the batched
+    workload's lattice): a point and a Z2 source in one batch converge, and
+    their true relative residuals, recomputed with the complex128 operator
+    (oracle-checked to 1e-13 at 16^3x32), are <= 1e-10."""
+    from tfleonardym_amd import dirac, dirac_solver, lattice
+    geometry = (24, 24, 24, 48)
+    print(f"case batched half CG geometry={geometry}", flush=True)
+    U = fields.gauge_field(geometry, 3, seed=20250, kind="near_identity", eps=0.25)
+    op = dirac.DiracWilsonOperator(lattice.Configuration(U, geometry, 3, 4), 0.1335, True,
+                                   representation="fundamental")
+    srcs = [torch.from_numpy(fields.point_source(geometry, 4, 3, site=0, color=1, spin=2)).cuda(),
+            torch.from_numpy(fields.z2_noise(geometry, 4, 3, seed=20261)).cuda()]
+    A = dirac.SquareOperator(op)
+    s = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="half",
+                                       verbose=False)
+    xs = s.solve_batch(A, srcs, [0.0])
+    assert s.last_status == "converged", s.last_stats
+    for b, row in zip(srcs, xs):
+        r = b - A.apply(row[0])
+        assert (torch.linalg.vector_norm(r) / torch.linalg.vector_norm(b)).item() <= 1e-10
+
+
 def test_c1_plain_fp64_cg():
     """The CPU-runnable workload: 2D 24x16 SU(2), Haar links, kappa=-0.15,
     periodic x / antiperiodic t, FP64 CG on D^dag D to 1e-10."""
