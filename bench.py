@@ -217,13 +217,17 @@ def bytes_per_site_dslash(nc, ndim, wl, ws, link_rows=None):
 DSLASH_C3 = dict(geometry=(24, 24, 24, 48), nc=3, kappa=0.1335, eps=0.25, seed=20250)
 
 
+KERNEL_TIME_REPLAYS = 5
+
+
 def kernel_time_us(launch, reps):
-    """Average duration in us of one `launch()` (a single kernel) over `reps`
-    back-to-back launches.  The launches are captured into a HIP graph and
-    replayed, so host-side launch overhead (ctypes, descriptor validation, a
-    profiler's API interception) does not enter the figure; HIP events are
-    recorded on the stream the graph replays on.  Falls back to timing the
-    direct launches if the capture fails."""
+    """Duration in us of one `launch()` (a single kernel): the median over
+    KERNEL_TIME_REPLAYS timed runs of the average over `reps` back-to-back
+    launches.  The launches are captured into a HIP graph and replayed, so
+    host-side launch overhead (ctypes, descriptor validation, a profiler's API
+    interception) does not enter the figure; HIP events are recorded on the
+    stream the graph replays on.  Falls back to timing the direct launches if
+    the capture fails."""
     import torch
     for _ in range(3):
         launch()
@@ -243,15 +247,18 @@ def kernel_time_us(launch, reps):
         print(f"[bench] graph capture failed ({exc}); timing direct launches", file=sys.stderr, flush=True)
         graph = None
         torch.cuda.synchronize()
-    e0.record(stream)
-    if graph is not None:
-        graph.replay()
-    else:
-        for _ in range(reps):
-            launch()
-    e1.record(stream)
-    e1.synchronize()
-    return e0.elapsed_time(e1) * 1e3 / reps
+    runs = []
+    for _ in range(KERNEL_TIME_REPLAYS):
+        e0.record(stream)
+        if graph is not None:
+            graph.replay()
+        else:
+            for _ in range(reps):
+                launch()
+        e1.record(stream)
+        e1.synchronize()
+        runs.append(e0.elapsed_time(e1) * 1e3 / reps)
+    return float(np.median(runs))
 
 
 def time_dslash(op, src, reps):
