@@ -76,8 +76,8 @@ global_sum = mth_global_sum
 
 def set_translate(use_mpi, geometry, mpi_grid=None):
     """Install the single-process translate / tables / global_sum
-    (translate.py:9-36).  Domain decomposition over several GPUs is not part of
-    this build."""
+    (translate.py:9-36).  Domain decomposition over several GPUs is not
+    implemented: one process drives one GPU."""
     global lookup_tables
     if use_mpi:
         raise NotImplementedError("multi-process domain decomposition is not built (one GPU)")
