@@ -189,6 +189,30 @@ def test_batch_tuned_tile(mode, monkeypatch):
         assert rel(xt, xf.cpu().numpy()) < 1e-8
 
 
+@pytest.mark.parametrize("hif", ["1", "2"])
+def test_half_batch_hops_in_flight(hif, monkeypatch):
+    """The 16-bit batched CG with one or two hops in flight per wave (forced
+    on the one-RHS tile, batch tuning off): every source reaches 1e-10
+    (oracle-recomputed) and agrees with the default launch to 1e-8."""
+    from tfleonardym_amd import dirac, dirac_solver
+    geometry = (8, 4, 4, 16)
+    print(f"case half batch hif={hif}", flush=True)
+    op, oop = setup(geometry, 3, 0.13)
+    srcs = [fields.gaussian_spinor(geometry, 4, 3, seed=95), fields.z2_noise(geometry, 4, 3, seed=96)]
+    s = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="half",
+                                       verbose=False)
+    ref = s.solve_batch(dirac.SquareOperator(op), srcs, [0.0])
+    monkeypatch.setenv("LYM_NO_BATCH_TUNE", "1")
+    monkeypatch.setenv("LYM_HIF", hif)
+    assert max(op.tile_info("f16")["hif"], 1) == int(hif)
+    got = s.solve_batch(dirac.SquareOperator(op), srcs, [0.0])
+    assert s.last_status == "converged", s.last_stats
+    for b, r0, r1 in zip(srcs, ref, got):
+        x = r1[0].cpu().numpy()
+        assert osv.true_residual(oop.apply_square, b, x, 0.0) <= 1e-10
+        assert rel(x, r0[0].cpu().numpy()) < 1e-8
+
+
 @pytest.mark.parametrize("geometry,nc", [((4, 4, 4, 4), 2), ((4, 4), 3)])
 def test_mixed_cg_adjoint(geometry, nc):
     """One shift, mixed precision, adjoint representation (Nc^2 colour
