@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define LYM_ABI_VERSION 3
+#define LYM_ABI_VERSION 4
 
 enum {
   LYM_OK = 0,
@@ -77,11 +77,15 @@ typedef struct lym_solve_stats {
   int32_t status;      /* 1 converged, 2 max iterations, 3 breakdown */
   int32_t iterations;  /* iterations of the (base) recurrence */
   int32_t nshift;
-  int32_t restarts;    /* reliable updates / refinement cycles */
+  int32_t restarts;    /* reliable updates + defect-correction solves of this RHS */
   double final_rr;     /* last iterated <r,r> of the base system */
   double seconds;      /* wall time of the call (host clock, synchronised) */
   int32_t conv_iter[64];
-  double true_rel_resid[64]; /* ||b - (A+s)x|| / ||b||, FP64, when computed */
+  double true_rel_resid[64];    /* ||b - (A+s)x|| / ||b||, FP64, when computed */
+  double initial_rel_resid[64]; /* the same before any defect correction */
+  int32_t refine_cycles[64];    /* defect-correction solves of each shift */
+  int32_t refine_iterations;    /* iterations of those solves (not in `iterations`) */
+  int32_t precision_phase;      /* fast CG-M: last Krylov storage, 0 FP64, 1 complex64, 2 16-bit */
 } lym_solve_stats;
 
 /* ---- runtime --------------------------------------------------------- */

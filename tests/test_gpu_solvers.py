@@ -115,6 +115,64 @@ def test_mixed_cg_reliable_updates(shift, mr, monkeypatch):
         assert rel(xn, single[0].cpu().numpy()) < 1e-8
 
 
+@pytest.mark.parametrize("geometry,nc,tune", [((8, 4, 4, 16), 3, "1e-2,1e-4,0.5"), ((4, 4, 4, 8), 2, "1e-2,1e-4,0.5"),
+                                                ((8, 8), 3, "1e-2,1e-4,0.5"), ((8, 4, 4, 16), 3, "1e-4,1e-7,0.5")])
+def test_half_storage_multishift(geometry, nc, tune, monkeypatch):
+    """Multishift CG with 16-bit Krylov storage (LYM_PREC_F16, several
+    shifts): FP64 vectors, then complex64 below the first threshold, then
+    int16 + per-site norms below the second, shift solutions in FP64.  The
+    thresholds are raised (LYM_CGM_TUNE) so that all three phases run on a
+    small lattice, and also left at their defaults.  Every shift reaches
+    1e-10 (true residual recomputed by the oracle), matches the oracle's FP64
+    CG-M to 1e-8, and the batched solve of two sources matches their single
+    solves."""
+    from tfleonardym_amd import dirac, dirac_solver
+    monkeypatch.setenv("LYM_CGM_TUNE", tune)
+    print(f"case half multishift geometry={geometry} nc={nc} tune={tune}", flush=True)
+    op, oop = setup(geometry, nc, 0.13)
+    ns = 2 if len(geometry) == 2 else 4
+    srcs = [fields.gaussian_spinor(geometry, ns, nc, seed=80), fields.z2_noise(geometry, ns, nc, seed=81)]
+    shifts = fields.log_shifts(1e-3, 5.0, 6)
+    s = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="half",
+                                       verbose=False)
+    sols = s.solve_batch(dirac.SquareOperator(op), srcs, shifts)
+    assert s.last_status == "converged", s.last_stats
+    if tune.startswith("1e-2"):
+        # the 2D lattice converges in ~33 iterations, before the 16-bit
+        # phase (phases change at chunk boundaries): complex64 is reached
+        want = 2 if len(geometry) == 4 else 1
+        assert all(st["precision_phase"] >= want for st in s.last_stats), s.last_stats
+    for b, row in zip(srcs, sols):
+        ref, _ = osv.cgm(oop.apply_square, b, shifts, tol=1e-12)
+        single = s.solve(dirac.SquareOperator(op), b, shifts)
+        for sh, x, xr, x1 in zip(shifts, row, ref, single):
+            xn = x.cpu().numpy()
+            assert osv.true_residual(oop.apply_square, b, xn, sh) <= 1e-10, sh
+            assert rel(xn, xr) < 1e-8, sh
+            assert rel(xn, x1.cpu().numpy()) < 1e-8, sh
+
+
+def test_half_multishift_zero_source(monkeypatch):
+    """A zero source next to a Gaussian one in a 16-bit multishift batch: the
+    zero source is solved by x = 0 at once and does not hold back the
+    precision switches of the other."""
+    from tfleonardym_amd import dirac, dirac_solver
+    monkeypatch.setenv("LYM_CGM_TUNE", "1e-2,1e-4,0.5")
+    geometry = (4, 4, 4, 8)
+    op, oop = setup(geometry, 3, 0.13)
+    b = fields.gaussian_spinor(geometry, 4, 3, seed=82)
+    srcs = [np.zeros_like(b), b]
+    shifts = [1.0, 0.1, 0.01]
+    s = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="half",
+                                       verbose=False)
+    sols = s.solve_batch(dirac.SquareOperator(op), srcs, shifts)
+    assert s.last_status == "converged", s.last_stats
+    assert all(float(x.abs().max()) == 0.0 for x in sols[0])
+    assert s.last_stats[1]["precision_phase"] == 2
+    for sh, x in zip(shifts, sols[1]):
+        assert osv.true_residual(oop.apply_square, b, x.cpu().numpy(), sh) <= 1e-10
+
+
 @pytest.mark.parametrize("geometry,nc", [((8, 4, 4, 16), 3), ((4, 4, 4, 8), 2), ((8, 8), 3)])
 def test_half_storage_cg(geometry, nc):
     """One shift with half-precision Krylov storage (per-site norms) and FP64

@@ -15,7 +15,7 @@ PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 REP_FUNDAMENTAL, REP_ADJOINT = 0, 1
 MODE_PROJECTED, MODE_SIMPLE = 0, 1
 LINKS_FULL, LINKS_SU = 0, 1
-ABI_VERSION = 3  # LYM_ABI_VERSION of include/lym.h
+ABI_VERSION = 4  # LYM_ABI_VERSION of include/lym.h
 
 OK = 0
 NOT_CONVERGED = 1
@@ -38,7 +38,7 @@ class LymWilson(ctypes.Structure):
         ("links_f32", ctypes.c_void_p),
         ("links_right_f32", ctypes.c_void_p),
         ("link_format", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("precision_phase", ctypes.c_int32),
     ]
 
 
@@ -68,6 +68,10 @@ class LymSolveStats(ctypes.Structure):
         ("seconds", ctypes.c_double),
         ("conv_iter", ctypes.c_int32 * 64),
         ("true_rel_resid", ctypes.c_double * 64),
+        ("initial_rel_resid", ctypes.c_double * 64),
+        ("refine_cycles", ctypes.c_int32 * 64),
+        ("refine_iterations", ctypes.c_int32),
+        ("precision_phase", ctypes.c_int32),
     ]
 
     def as_dict(self, nshift=None):
@@ -80,6 +84,10 @@ class LymSolveStats(ctypes.Structure):
             "seconds": float(self.seconds),
             "conv_iter": [int(self.conv_iter[i]) for i in range(n)],
             "true_rel_resid": [float(self.true_rel_resid[i]) for i in range(n)],
+            "initial_rel_resid": [float(self.initial_rel_resid[i]) for i in range(n)],
+            "refine_cycles": [int(self.refine_cycles[i]) for i in range(n)],
+            "refine_iterations": int(self.refine_iterations),
+            "precision_phase": int(self.precision_phase),
         }
 
 

@@ -93,9 +93,9 @@ __device__ inline void cgm_fin_b_fast(SolverState* st, double rrn) {
     if (!st->active[s]) continue;
     const double q = st->zeta_new[s] / st->zeta[s];
     const double bs = beta * q * q;
-    st->h_alpha[j][s] = st->alpha_s[s];
-    st->h_zeta[j][s] = st->zeta_new[s];
-    st->h_beta[j][s] = bs;
+    st->h_alpha[s][j] = st->alpha_s[s];
+    st->h_zeta[s][j] = st->zeta_new[s];
+    st->h_beta[s][j] = bs;
     st->n_upd[s] = j + 1;
     st->zeta_old[s] = st->zeta[s];
     st->zeta[s] = st->zeta_new[s];
@@ -140,9 +140,10 @@ __device__ inline void cgm_fin_b_fast(SolverState* st, double rrn) {
 
 // Mixed CG after the low-precision <r_{k+1}, r_{k+1}>: beta = rrn / rr and
 // p = r + beta p next (PH_KP), unless a reliable update is due: the iterated
-// residual fell by delta below its largest value since the last update, or
+// residual fell by delta below the true residual of the last update, or
 // passed the stopping test, or the iteration budget is spent (PH_RU; rr keeps
-// <r_k, r_k> so the update can form beta from the true residual).
+// <r_k, r_k> so the update can form beta from the true residual).  rr_max is
+// the true residual of the last update (|b|^2 before the first).
 __device__ inline void cg_fin_b_rel(SolverState* st, double rrn) {
   st->iter += 1;
   st->x_pending = 1;  // x_lo += alpha p happens with the next p update (or the reliable update)

@@ -9,7 +9,7 @@
 namespace lym {
 
 constexpr int kMaxShifts = 64;
-constexpr int kMaxBlock = 16;  // deferred multishift updates: iterations per block
+constexpr int kMaxBlock = 64;  // deferred multishift updates: iterations per block
 
 enum Phase : int {
   PH_K1 = 0,    // t = Q p
@@ -53,20 +53,24 @@ struct SolverState {
   int conv_iter[kMaxShifts];
   // ---- fast CG-M: deferred shift updates over blocks of `mblk` iterations.
   // Step j of the current block used r^{k0+j+1} (ring slot (k0+j+1) % (mblk+1))
-  // and the scalars below; n_upd[s] = steps recorded for shift s.
+  // and the scalars below; n_upd[s] = steps recorded for shift s.  The
+  // history is stored shift-major ([shift][step]) so the update kernel reads
+  // one shift's steps as consecutive scalars.
   int mblk;
   int blk_j;         // steps recorded in the current block
-  int precision_phase;  // 0 = FP64 Krylov vectors, 1 = low precision
+  int precision_phase;  // 0 = FP64 Krylov vectors, 1 = complex64, 2 = 16-bit
   int switch_counted;   // this RHS has been counted in the batch's switch tally
-  double switch_rr;  // absolute threshold on rr for want_switch
+  double switch_rr;  // absolute threshold on rr for want_switch (next phase)
+  double switch2_rr; // threshold of the phase after that (0: none)
   int n_upd[kMaxShifts];
-  double h_alpha[kMaxBlock][kMaxShifts];
-  double h_zeta[kMaxBlock][kMaxShifts];
-  double h_beta[kMaxBlock][kMaxShifts];
+  double h_alpha[kMaxShifts][kMaxBlock];
+  double h_zeta[kMaxShifts][kMaxBlock];
+  double h_beta[kMaxShifts][kMaxBlock];
   // ---- mixed-precision CG with reliable updates (one shift): the low-
-  // precision recurrence runs until rr < delta2 * rr_max (rr_max = largest
-  // rr since the last update), then x_hi += x_lo and r = b - (A + s) x_hi in
-  // FP64 replace the iterated residual.
+  // precision recurrence runs until rr < delta2 * rr_max (rr_max = the true
+  // residual |r|^2 of the last reliable update, or |b|^2 before the first),
+  // then x_hi += x_lo and r = b - (A + s) x_hi in FP64 replace the iterated
+  // residual.
   double rr_max;
   double delta2;
   double tol2_true;  // target on the true rr (absolute after the init scaling)

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -117,6 +118,17 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
     sums[1] = s[1];
     sums[2] = s[2];
   }
+  // kinds whose per-shift recurrences run on every thread of the block
+  switch (kind) {
+    case FK_INIT:
+      if ((int)threadIdx.x < st->nshift) st->tol2_s[threadIdx.x] *= sums[0];
+      break;
+
+    case FK_FAST_K3ADV:
+      if ((int)threadIdx.x < st->nshift) st->n_upd[threadIdx.x] = 0;
+      break;
+    default: break;
+  }
   if (threadIdx.x != 0) return;
   switch (kind) {
     case FK_SUM:
@@ -125,8 +137,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
     case FK_INIT:
       st->bb = sums[0];
       st->rr = sums[0];
-      for (int k = 0; k < st->nshift; ++k) st->tol2_s[k] *= sums[0];
       st->switch_rr *= sums[0];
+      st->switch2_rr *= sums[0];
       st->tol2_true *= sums[0];
       st->rr_max = sums[0];
       if (!(sums[0] > 0.0)) {
@@ -140,7 +152,17 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
         }
       }
       break;
-    case FK_CGM_A: cgm_fin_a(st, sums[0], sums[1]); break;
+    case FK_CGM_A:
+      cgm_fin_a(st, sums[0], sums[1]);
+      // a breakdown ends the solve here, not at a block flush: count it
+      if (ctl && st->phase == PH_DONE) {
+        atomicAdd(&ctl->ndone, 1);
+        if (!st->switch_counted) {
+          st->switch_counted = 1;
+          atomicAdd(&ctl->nswitch, 1);
+        }
+      }
+      break;
     case FK_CGM_B: cgm_fin_b(st, sums[0]); break;
     case FK_CGM_BF:
       cgm_fin_b_fast(st, sums[0]);
@@ -152,7 +174,6 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
     case FK_CGM_K3ADV: st->phase = st->final_k3 ? PH_DONE : PH_K1; break;
     case FK_FAST_K3ADV:
       st->blk_j = 0;
-      for (int k = 0; k < st->nshift; ++k) st->n_upd[k] = 0;
       st->phase = st->final_k3 ? PH_DONE : PH_KP;
       if (ctl && st->phase == PH_DONE) {
         atomicAdd(&ctl->ndone, 1);
@@ -169,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
     case FK_CG_RU:
       if (kind == FK_CG_A) cgm_fin_a(st, sums[0], sums[1]);
       else if (kind == FK_CG_B) cg_fin_b_rel(st, sums[0]);
-      else cg_fin_ru(st, sums[0], sums[1], sums[2]);
+      else if (kind == FK_CG_RU) cg_fin_ru(st, sums[0], sums[1], sums[2]);
       if (ctl && st->phase == PH_DONE) atomicAdd(&ctl->ndone, 1);
       break;
   }
@@ -277,65 +298,102 @@ __global__ __launch_bounds__(kBlock) void k_fast_kp(const SolverState* st, int64
   p += rhs * n;
   const E* r = ring + (int64_t)(st->iter % nslot) * n;
   const R b = (R)st->beta_old;
-  const R ar = (R)st->kp_a_re, ai = (R)st->kp_a_im;
-  if (ar == R(1) && ai == R(0)) {
-    LYM_GRID_STRIDE(i, n) {
-      const E rv = r[i];
-      E q = p[i];
-      q.x = rv.x + b * q.x;
-      q.y = rv.y + b * q.y;
-      p[i] = q;
-    }
-  } else {
-    // after a reliable update: p = kp_a r + beta p
-    LYM_GRID_STRIDE(i, n) {
-      const E rv = r[i];
-      E q = p[i];
-      const R qx = ar * rv.x - ai * rv.y + b * q.x;
-      const R qy = ar * rv.y + ai * rv.x + b * q.y;
-      q.x = qx;
-      q.y = qy;
-      p[i] = q;
-    }
+  LYM_GRID_STRIDE(i, n) {
+    const E rv = r[i];
+    E q = p[i];
+    q.x = rv.x + b * q.x;
+    q.y = rv.y + b * q.y;
+    p[i] = q;
+  }
+}
+
+// The same for 16-bit storage (per-site norms): one thread per site.
+// Ring slot q of RHS i: data + (i * nslot + q) * NCOMP * V, norms + (i * nslot + q) * V.
+template <int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_fast_kp16(const SolverState* st, int64_t V, Field ring, int nslot,
+                                                      Field p) {
+  const int rhs = blockIdx.y;
+  st += rhs;
+  if (st->phase != PH_KP) return;
+  const int64_t slot = (int64_t)rhs * nslot + st->iter % nslot;
+  const SpinorView<P16> rv{(short2*)ring.data + slot * NCOMP * V, ring.norm + slot * V, V};
+  const SpinorView<P16> pv{(short2*)p.data + (int64_t)rhs * NCOMP * V, p.norm + (int64_t)rhs * V, V};
+  const float b = (float)st->beta_old;
+  for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < V; s += (int64_t)gridDim.x * kBlock) {
+    cpx<float> a[NCOMP], q[NCOMP];
+    rv.template load<NCOMP>(s, a);
+    pv.template load<NCOMP>(s, q);
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) q[c] = mk<float>(fmad(b, q[c].re, a[c].re), fmad(b, q[c].im, a[c].im));
+    pv.template store<NCOMP>(s, q);
   }
 }
 
 // Deferred shift updates of one block: for every shift, apply its recorded
 // steps j < n_upd[s]:  x_s += a_j p_s ;  p_s = z_j r^{k0+j+1} + b_j p_s
-// (x_s, p_s in FP64; each residual element read once per block).
-template <int PR, int MB>
-__global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int64_t n, const Elem<PR>* __restrict__ ring,
+// with x_s, p_s in FP64.  One thread per element (component plane c, site);
+// the block's residuals r^{k0+1..k0+nb} at that element are loaded once into
+// registers, then the active shifts stream their x_s, p_s through them.  Per
+// block the kernel moves nb residuals plus 4 vectors per active shift, so
+// the shift traffic per iteration falls with the block length MB, at the
+// cost of 2 MB (float) or 4 MB (double) residual registers.
+//   k_fast_k3 (FP64 ring, MB = 16): 64 VGPRs, occupancy 8; the shifts are
+//     updated one after the other, their loads kept in flight by occupancy.
+//   k_fast_k3s (complex64 / 16-bit ring decoded to float, MB = 32): about
+//     100 VGPRs, occupancy 4; the next shift's pair is loaded while the
+//     current one is updated.
+// Work item m covers site 64 * (m / 64 / ncomp) + m % 64 of plane
+// (m / 64) % ncomp: consecutive waves take the planes of the same 64 sites,
+// so the per-site norms of a 16-bit ring are re-read from cache.
+// (Measured on MI355X at 32^3x64 with 24 shifts: complex128 blocks of 32 or
+// complex64 blocks of 64, which need 128 residual registers, ran at
+// occupancy 1-3 and gave no gain; 16 and 32 are the fastest.)
+#define LYM_K3_ITEM(m, ncomp, V, c, site, i)              \
+  const int64_t grp_ = (m) >> 6;                          \
+  const int c = (int)(grp_ % (ncomp));                    \
+  const int64_t site = (grp_ / (ncomp)) * 64 + ((m) & 63); \
+  if (site >= (V)) continue;                              \
+  const int64_t i = (int64_t)c * (V) + site;
+
+constexpr int kK3Block64 = 16;
+constexpr int kK3BlockLo = 32;
+
+__global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int ncomp, int64_t V, Field ring,
                                                     int64_t ring_stride, int nslot, const Field* xs, const Field* ps,
                                                     int nshift) {
+  constexpr int MB = kK3Block64;
   const int rhs = blockIdx.y;
   st += rhs;
   if (st->phase != PH_K3) return;
-  ring += rhs * ring_stride;
+  const double2* rg = (const double2*)ring.data + rhs * ring_stride;
   xs += rhs * nshift;
   ps += rhs * nshift;
   const int nb = st->blk_j;
   const int k0 = st->iter - nb;
-  LYM_GRID_STRIDE(i, n) {
+  const int64_t n = (int64_t)ncomp * V;
+  const int64_t nwork = ((V + 63) / 64) * ncomp * 64;
+  LYM_GRID_STRIDE(m, nwork) {
+    LYM_K3_ITEM(m, ncomp, V, c, site, i)
     double2 r[MB];
 #pragma unroll
     for (int j = 0; j < MB; ++j) {
       r[j] = make_double2(0.0, 0.0);
-      if (j < nb) r[j] = to64(ring[(int64_t)((k0 + j + 1) % nslot) * n + i]);
+      if (j < nb) r[j] = rg[(int64_t)((k0 + j + 1) % nslot) * n + i];
     }
     for (int k = 0; k < nshift; ++k) {
-      const int m = st->n_upd[k];
-      if (m == 0) continue;
+      const int mk = st->n_upd[k];
+      if (mk == 0) continue;
       double2* xp = (double2*)xs[k].data;
       double2* pp = (double2*)ps[k].data;
       double2 x = xp[i], p = pp[i];
 #pragma unroll
       for (int j = 0; j < MB; ++j) {
-        if (j < m) {
-          const double a = st->h_alpha[j][k], z = st->h_zeta[j][k], b = st->h_beta[j][k];
-          x.x += a * p.x;
-          x.y += a * p.y;
-          p.x = z * r[j].x + b * p.x;
-          p.y = z * r[j].y + b * p.y;
+        if (j < mk) {
+          const double a = st->h_alpha[k][j], z = st->h_zeta[k][j], b = st->h_beta[k][j];
+          x.x = fmad(a, p.x, x.x);
+          x.y = fmad(a, p.y, x.y);
+          p.x = fmad(z, r[j].x, b * p.x);
+          p.y = fmad(z, r[j].y, b * p.y);
         }
       }
       xp[i] = x;
@@ -344,15 +402,109 @@ __global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int64
   }
 }
 
-// precision switch at a block boundary: low-precision copies of the current
-// residual (FP64 ring slot iter % nslot64 -> low ring slot iter % nslot_lo)
-// and of the live direction; later blocks run mblk_lo iterations
+template <int PR>
+__global__ __launch_bounds__(kBlock) void k_fast_k3s(const SolverState* st, int ncomp, int64_t V, Field ring,
+                                                     int64_t ring_stride, int nslot, const Field* xs, const Field* ps,
+                                                     int nshift) {
+  constexpr int MB = kK3BlockLo;
+  using E = Elem<PR>;
+  const int rhs = blockIdx.y;
+  st += rhs;
+  if (st->phase != PH_K3) return;
+  const E* rg = (const E*)ring.data + rhs * ring_stride;
+  const float* rn = ring.norm ? ring.norm + (int64_t)rhs * nslot * V : nullptr;
+  xs += rhs * nshift;
+  ps += rhs * nshift;
+  const int nb = st->blk_j;
+  const int k0 = st->iter - nb;
+  const int64_t n = (int64_t)ncomp * V;
+  const int64_t nwork = ((V + 63) / 64) * ncomp * 64;
+  int kfirst = 0;
+  while (kfirst < nshift && st->n_upd[kfirst] == 0) ++kfirst;
+  if (kfirst >= nshift) return;
+  LYM_GRID_STRIDE(m, nwork) {
+    LYM_K3_ITEM(m, ncomp, V, c, site, i)
+    float rr[MB], ri[MB];
+#pragma unroll
+    for (int j = 0; j < MB; ++j) {
+      rr[j] = 0.f;
+      ri[j] = 0.f;
+      if (j < nb) {
+        const int q = (k0 + j + 1) % nslot;
+        const E e = rg[(int64_t)q * n + i];
+        if constexpr (PR == P16) {
+          const float sc = rn[(int64_t)q * V + site];
+          rr[j] = (float)e.x * sc;
+          ri[j] = (float)e.y * sc;
+        } else {
+          rr[j] = e.x;
+          ri[j] = e.y;
+        }
+      }
+    }
+    int k = kfirst;
+    double2 x = ((const double2*)xs[k].data)[i];
+    double2 p = ((const double2*)ps[k].data)[i];
+    while (k < nshift) {
+      int kn = k + 1;
+      while (kn < nshift && st->n_upd[kn] == 0) ++kn;
+      double2 xn = x, pn = p;
+      if (kn < nshift) {
+        xn = ((const double2*)xs[kn].data)[i];
+        pn = ((const double2*)ps[kn].data)[i];
+      }
+      // converted per use: otherwise the compiler hoists the conversions out
+      // of the shift loop and keeps 4 MB doubles live instead of 2 MB floats
+#pragma unroll
+      for (int j = 0; j < MB; ++j) asm volatile("" : "+v"(rr[j]), "+v"(ri[j]));
+      const int mk = st->n_upd[k];
+      const double* ha = st->h_alpha[k];
+      const double* hz = st->h_zeta[k];
+      const double* hb = st->h_beta[k];
+#pragma unroll
+      for (int j = 0; j < MB; ++j) {
+        if (j < mk) {
+          const double a = ha[j], z = hz[j], b = hb[j];
+          x.x = fmad(a, p.x, x.x);
+          x.y = fmad(a, p.y, x.y);
+          p.x = fmad(z, (double)rr[j], b * p.x);
+          p.y = fmad(z, (double)ri[j], b * p.y);
+        }
+      }
+      ((double2*)xs[k].data)[i] = x;
+      ((double2*)ps[k].data)[i] = p;
+      k = kn;
+      x = xn;
+      p = pn;
+    }
+  }
+}
+
+// Shared state transition of a precision switch (block (0, rhs), thread 0):
+// the next phase, its block length and threshold; the batch tally restarts
+// from the RHS that have finished.
+__device__ inline void fast_switch_state(SolverState* st, BatchCtl* ctl, int mblk_lo) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  st->precision_phase += 1;
+  st->mblk = mblk_lo;
+  st->want_switch = 0;
+  st->switch_counted = st->phase == PH_DONE ? 1 : 0;
+  st->switch_rr = st->switch2_rr;
+  st->switch2_rr = 0.0;
+  if (blockIdx.y == 0) ctl->nswitch = ctl->ndone;
+}
+
+// precision switch FP64 -> complex64 at a block boundary: copies of the
+// current residual (FP64 ring slot iter % nslot64 -> low ring slot
+// iter % nslot_lo) and of the live direction; later blocks run mblk_lo
+// iterations
 template <int PL>
-__global__ __launch_bounds__(kBlock) void k_fast_switch(SolverState* st, int64_t n, const double2* __restrict__ ring64,
-                                                        int64_t ring_stride64, int nslot64,
-                                                        Elem<PL>* __restrict__ ringlo, int64_t ring_stride_lo,
-                                                        int nslot_lo, const double2* __restrict__ p64,
-                                                        Elem<PL>* __restrict__ plo, int mblk_lo) {
+__global__ __launch_bounds__(kBlock) void k_fast_switch(SolverState* st, BatchCtl* ctl, int64_t n,
+                                                        const double2* __restrict__ ring64, int64_t ring_stride64,
+                                                        int nslot64, Elem<PL>* __restrict__ ringlo,
+                                                        int64_t ring_stride_lo, int nslot_lo,
+                                                        const double2* __restrict__ p64, Elem<PL>* __restrict__ plo,
+                                                        int mblk_lo) {
   const int rhs = blockIdx.y;
   st += rhs;
   ring64 += rhs * ring_stride64;
@@ -365,10 +517,32 @@ __global__ __launch_bounds__(kBlock) void k_fast_switch(SolverState* st, int64_t
     ringlo[offlo + i] = from64<Elem<PL>>(ring64[off64 + i]);
     plo[i] = from64<Elem<PL>>(p64[i]);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->precision_phase = 1;
-    st->mblk = mblk_lo;
+  fast_switch_state(st, ctl, mblk_lo);
+}
+
+// precision switch complex64 -> 16-bit storage (one thread per site)
+template <int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_fast_switch16(SolverState* st, BatchCtl* ctl, int64_t V,
+                                                          const float2* __restrict__ ring32, int nslot32,
+                                                          Field ring16, int nslot16, const float2* __restrict__ p32,
+                                                          Field p16, int mblk_lo) {
+  const int rhs = blockIdx.y;
+  st += rhs;
+  const int64_t n = (int64_t)NCOMP * V;
+  const int64_t q32 = (int64_t)rhs * nslot32 + st->iter % nslot32;
+  const int64_t q16 = (int64_t)rhs * nslot16 + st->iter % nslot16;
+  const SpinorView<P32> ri{const_cast<float2*>(ring32) + q32 * n, nullptr, V};
+  const SpinorView<P32> pi{const_cast<float2*>(p32) + rhs * n, nullptr, V};
+  const SpinorView<P16> ro{(short2*)ring16.data + q16 * n, ring16.norm + q16 * V, V};
+  const SpinorView<P16> po{(short2*)p16.data + rhs * n, p16.norm + (int64_t)rhs * V, V};
+  for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < V; s += (int64_t)gridDim.x * kBlock) {
+    cpx<float> v[NCOMP];
+    ri.template load<NCOMP>(s, v);
+    ro.template store<NCOMP>(s, v);
+    pi.template load<NCOMP>(s, v);
+    po.template store<NCOMP>(s, v);
   }
+  fast_switch_state(st, ctl, mblk_lo);
 }
 
 // ---- mixed CG with reliable updates: site kernels (grid.y = RHS) -----------
@@ -585,15 +759,42 @@ static Field take_field(Carver& c, const OpDesc& d, int prec) {
   return f;
 }
 
-// iterations per deferred shift-update block (each block reads and writes
-// every shift's FP64 x_s, p_s once).  16 in both phases: against 8 while the
-// Krylov vectors are FP64 it costs the precision switch a few iterations'
-// wait for the block boundary and saves half the FP64-phase shift traffic
-// (measured: 16^3x32 with 12 shifts 2.5% faster, 32^3x64 with 24 shifts 10%);
-// 32 in the FP32 phase gained under 1% for twice the residual ring.
-constexpr int kFastBlock64 = 16;
-constexpr int kFastBlock32 = 16;
-static_assert(kFastBlock64 <= kMaxBlock && kFastBlock32 <= kMaxBlock, "block longer than the recorded history");
+// Iterations per deferred shift-update block.  A block reads its residuals
+// once and every active shift's FP64 x_s, p_s once (4 vectors per shift), so
+// the shift traffic per iteration falls as 1/m; the update kernel holds the
+// block's residuals of one element in registers (m complex numbers: 128
+// VGPRs for 32 complex128 or 64 complex64 / 16-bit ones).
+// (block lengths: CgmTuning::mblk_64 and mblk_lo, at most kMaxBlock; each
+// ring has one slot more than its block is long)
+
+// Thresholds of the fast multishift CG (relative to |b|): the base residual
+// below which complex64 Krylov vectors (switch32) and 16-bit storage
+// (switch16) take over, and the iterated tolerance as a fraction of the
+// target (the FP64 true residual decides at the end).  A rounding error of
+// relative size e made while the base residual is r shows up in the shifted
+// true residuals at about e r, summed over the iterations of the phase:
+// complex64 from 1e-4 and 16-bit from 1e-7 keep that sum below the target.
+// LYM_CGM_TUNE="switch32,switch16,iter_factor" overrides them (tuning runs).
+struct CgmTuning {
+  double switch32 = 1e-4;
+  double switch16 = 1e-7;
+  double iter_factor = 0.5;
+  int mblk_lo = kK3BlockLo;  // block length of the complex64 / 16-bit phases
+  int mblk_64 = kK3Block64;  // block length of the FP64 phase
+};
+static CgmTuning cgm_tuning() {
+  CgmTuning t;
+  const char* v = std::getenv("LYM_CGM_TUNE");
+  if (v && *v) {
+    double a = 0, b = 0, c = 0;
+    if (std::sscanf(v, "%lf,%lf,%lf", &a, &b, &c) == 3 && a > 0 && b > 0 && c > 0 && c <= 1) {
+      t.switch32 = a;
+      t.switch16 = b;
+      t.iter_factor = c;
+    }
+  }
+  return t;
+}
 
 struct Parts {
   SolverState* st;  // nrhs states
@@ -609,9 +810,10 @@ struct Parts {
   Field x_lo[kMaxShifts];
   Field p[kMaxShifts];  // plain CG-M search directions
   // fast CG-M (mixed), every field batched over nrhs RHS:
-  Field ring64, ring32;     // nrhs x (kFastBlock64 + 1), nrhs x (kFastBlock32 + 1) slots
-  Field plive64, plive32;   // live base direction
-  Field tb64, tb32;         // t = Q p
+  Field ring64, ring32, ring16;   // nrhs x nslot64 / nslot_lo slots
+  int nslot64 = 0, nslot_lo = 0;
+  Field plive64, plive32, plive16;  // live base direction
+  Field tb64, tb32, tb16;           // t = Q p
   Field ps64;               // nrhs x nshift FP64 shift directions
   // mixed CG with reliable updates (one shift), every field nrhs x batched:
   // residual, direction, Q p (complex64 or 16-bit storage), solution since
@@ -665,13 +867,26 @@ static size_t carve(const OpDesc& d, int prec, int nshift, int nrhs, char* base,
       P.rl_x64 = take_batch(P64);
       P.rl_t64 = take_batch(P64);
     } else {
-      P.ring64.data = c.take(field_bytes(d, P64) * (kFastBlock64 + 1) * nrhs);
-      P.ring32.data = c.take(field_bytes(d, P32) * (kFastBlock32 + 1) * nrhs);
+      const CgmTuning tun = cgm_tuning();
+      P.nslot64 = tun.mblk_64 + 1;
+      P.nslot_lo = tun.mblk_lo + 1;
+      P.ring64.data = c.take(field_bytes(d, P64) * P.nslot64 * nrhs);
+      P.ring32.data = c.take(field_bytes(d, P32) * P.nslot_lo * nrhs);
       P.plive64.data = c.take(field_bytes(d, P64) * nrhs);
       P.plive32.data = c.take(field_bytes(d, P32) * nrhs);
       P.tb64.data = c.take(field_bytes(d, P64) * nrhs);
       P.tb32.data = c.take(field_bytes(d, P32) * nrhs);
       P.ps64.data = c.take(field_bytes(d, P64) * nshift * nrhs);
+      if (prec == P16) {
+        // third phase: 16-bit storage with per-site norms
+        const size_t nb = (size_t)d.lat.V * sizeof(float);
+        P.ring16.data = c.take(field_bytes(d, P16) * P.nslot_lo * nrhs);
+        P.ring16.norm = (float*)c.take(nb * P.nslot_lo * nrhs);
+        P.plive16.data = c.take(field_bytes(d, P16) * nrhs);
+        P.plive16.norm = (float*)c.take(nb * nrhs);
+        P.tb16.data = c.take(field_bytes(d, P16) * nrhs);
+        P.tb16.norm = (float*)c.take(nb * nrhs);
+      }
     }
   }
   return c.off + 256;
@@ -967,24 +1182,32 @@ static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Fi
 }
 
 // Fast multishift CG over a batch of nrhs right-hand sides (one shift list).
-// Blocks of kFastBlock64 (FP64) or kFastBlock32 (FP32) iterations:
+// Blocks of m iterations (CgmTuning::mblk_64 in the FP64 phase, mblk_lo after):
 //   [ p = r + beta p ; t = Q p (|t|^2,|p|^2) ; r' = r - alpha (Q t + s0 p) (|r'|^2) ] x m
 //   then one deferred update of every shift's x_s, p_s (FP64) from the ring.
-// Krylov vectors are FP64 until every RHS has its base residual below
-// switch_tol |b| (or has finished), then `lowprec`.  xs[r * nshift + s] are the
-// caller's FP64 solutions.
+// Krylov vectors (r, p, t) are FP64 until every RHS has its base residual
+// below switch_tol |b| (or has finished), then complex64; with lowprec = P16
+// they move on to 16-bit storage (per-site norms, FP32 arithmetic) once every
+// base residual is below switch2_tol |b|.  A rounding error of a phase enters
+// the shifted solutions at the size of the residual it is made at, so the
+// thresholds keep each phase's contribution to the true residual below the
+// target.  xs[r * nshift + s] are the caller's FP64 solutions.
 static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void* const* xs, int nrhs, int nshift,
-                    const double* shifts, const double* tol_rel, int max_iter, double switch_tol, int lowprec,
+                    const double* shifts, const double* tol_rel, int max_iter, const CgmTuning& tun, int lowprec,
                     hipStream_t st, std::vector<SolverState>& out_states) {
+  const double switch_tol = tun.switch32, switch2_tol = tun.switch16;
+  const int mlo = tun.mblk_lo, m64 = tun.mblk_64;
   graph_new_solve();
   SolverState h;
   init_cgm_state(h, nshift, shifts, tol_rel, max_iter);
   h.phase = PH_KP;
-  h.mblk = kFastBlock64;
+  h.mblk = m64;
   h.switch_rr = switch_tol * switch_tol;
+  h.switch2_rr = lowprec == P16 ? switch2_tol * switch2_tol : 0.0;
   std::vector<SolverState> hs(nrhs, h);
   std::vector<Field> hx((size_t)nrhs * nshift), hp((size_t)nrhs * nshift), hb(nrhs);
   const int64_t n = field_elems(d);
+  const int64_t V = d.lat.V;
   for (int r = 0; r < nrhs; ++r) {
     hb[r].data = const_cast<void*>(bs[r]);
     for (int k = 0; k < nshift; ++k) {
@@ -1000,30 +1223,61 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   LYM_HIP(hipMemcpyAsync(P.bs_dev, hb.data(), sizeof(Field) * nrhs, hipMemcpyHostToDevice, st));
   const int gr = red_grid_flat(n);
   const int gf = flat_grid(n);
-  const int nslot64 = kFastBlock64 + 1, nslot32 = kFastBlock32 + 1;
+  const int gs = flat_grid(V);
+  const int nslot64 = P.nslot64, nslot32 = P.nslot_lo, nslot16 = P.nslot_lo;
+  if (m64 + 1 > nslot64 || mlo + 1 > nslot32) {
+    set_error("fast CG-M: block length changed since the workspace was sized");
+    return LYM_ERR_ARG;
+  }
   const int64_t ring_stride64 = (int64_t)nslot64 * n, ring_stride32 = (int64_t)nslot32 * n;
+  const int64_t ring_stride16 = (int64_t)nslot16 * n;
   hipLaunchKernelGGL(k_fast_init, dim3(gr, nrhs), dim3(kBlock), 0, st, n, P.bs_dev, (double2*)P.ring64.data,
                      ring_stride64, (double2*)P.plive64.data, P.xs_dev, P.ps_dev, nshift, P.partials);
   LYM_HIP(hipGetLastError());
   int rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
   if (rc) return rc;
+  if (lowprec == P16 && d.ncomp != 4 && d.ncomp != 6 && d.ncomp != 8 && d.ncomp != 12) {
+    set_error("16-bit multishift storage: fundamental representation only");
+    return LYM_ERR_UNSUPPORTED;
+  }
+  // site kernels of the 16-bit phase, instantiated per component count
+  auto by_ncomp = [&](auto f) -> int {
+    switch (d.ncomp) {
+      case 4: f(IC<4>{}); break;
+      case 6: f(IC<6>{}); break;
+      case 8: f(IC<8>{}); break;
+      case 12: f(IC<12>{}); break;
+      default:
+        set_error("16-bit multishift storage: unsupported number of spinor components");
+        return LYM_ERR_UNSUPPORTED;
+    }
+    LYM_HIP(hipGetLastError());
+    return LYM_OK;
+  };
 
   int prec = P64;
   auto chunk = [&](hipStream_t st) -> int {
-    const Field ring = prec == P64 ? P.ring64 : P.ring32;
-    const Field pl = prec == P64 ? P.plive64 : P.plive32;
-    const Field t = prec == P64 ? P.tb64 : P.tb32;
+    const Field ring = prec == P64 ? P.ring64 : prec == P32 ? P.ring32 : P.ring16;
+    const Field pl = prec == P64 ? P.plive64 : prec == P32 ? P.plive32 : P.plive16;
+    const Field t = prec == P64 ? P.tb64 : prec == P32 ? P.tb32 : P.tb16;
     const int gw = wilson_blocks(d, prec, nrhs);  // Wilson partials per RHS at this precision
-    const int mb = prec == P64 ? kFastBlock64 : kFastBlock32;
-    const int nslot = prec == P64 ? nslot64 : nslot32;
-    const int64_t ring_stride = prec == P64 ? ring_stride64 : ring_stride32;
+    const int mb = prec == P64 ? m64 : mlo;
+    const int nslot = prec == P64 ? nslot64 : prec == P32 ? nslot32 : nslot16;
+    const int64_t ring_stride = prec == P64 ? ring_stride64 : prec == P32 ? ring_stride32 : ring_stride16;
     for (int j = 0; j < mb; ++j) {
       if (prec == P64)
         hipLaunchKernelGGL(k_fast_kp<P64>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const double2*)ring.data,
                            ring_stride, nslot, (double2*)pl.data);
-      else
+      else if (prec == P32)
         hipLaunchKernelGGL(k_fast_kp<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const float2*)ring.data,
                            ring_stride, nslot, (float2*)pl.data);
+      else {
+        int e = by_ncomp([&](auto nc) {
+          hipLaunchKernelGGL(k_fast_kp16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, P.st, V, ring,
+                             nslot, pl);
+        });
+        if (e) return e;
+      }
       LYM_HIP(hipGetLastError());
       EpiArgs a1;
       a1.out = t;
@@ -1045,27 +1299,34 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
       if (e) return e;
     }
     if (prec == P64)
-      hipLaunchKernelGGL((k_fast_k3<P64, kFastBlock64>), dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
-                         (const double2*)ring.data, ring_stride, nslot, P.xs_dev, P.ps_dev, nshift);
+      hipLaunchKernelGGL(k_fast_k3, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, d.ncomp, V, ring, ring_stride, nslot,
+                         P.xs_dev, P.ps_dev, nshift);
+    else if (prec == P32)
+      hipLaunchKernelGGL(k_fast_k3s<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, d.ncomp, V, ring, ring_stride,
+                         nslot, P.xs_dev, P.ps_dev, nshift);
     else
-      hipLaunchKernelGGL((k_fast_k3<P32, kFastBlock32>), dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
-                         (const float2*)ring.data, ring_stride, nslot, P.xs_dev, P.ps_dev, nshift);
+      hipLaunchKernelGGL(k_fast_k3s<P16>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, d.ncomp, V, ring, ring_stride,
+                         nslot, P.xs_dev, P.ps_dev, nshift);
     LYM_HIP(hipGetLastError());
     return launch_fin(FK_FAST_K3ADV, P.st, nullptr, 0, 0, nullptr, st, nrhs, P.ctl);
   };
-  // poll the batch control block: (ndone, nswitch, nrhs, -)
+  // poll the batch control block: (ndone, nswitch, nrhs, -).  Each in-flight
+  // chunk remembers the precision it ran at: a tally read from a chunk issued
+  // before the last switch is stale and never triggers another one.
   int rce = ensure_sync();
   if (rce) return rce;
-  const int64_t max_chunks = (int64_t)max_iter / kFastBlock64 + 8;
+  const int64_t max_chunks = (int64_t)max_iter / 8 + 8;
   int64_t issued = 0, polled = 0;
+  int slot_prec[2] = {P64, P64};
   KeyBuf kb;
   kb.add('F').add(nrhs).add(nshift).add(P.st).add(P.xs_dev).add(P.ring64.data);
   const std::string key = op_key(d) + kb.s;
   auto issue = [&](int slot) -> int {
-    const int e = launch_chunk(key + (prec == P64 ? 'd' : 's'), st, chunk);
+    const int e = launch_chunk(key + (prec == P64 ? 'd' : prec == P32 ? 's' : 'h'), st, chunk);
     if (e) return e;
     LYM_HIP(hipMemcpyAsync(&g_sync.pinned[4 * slot], P.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
     LYM_HIP(hipEventRecord(g_sync.ev[slot], st));
+    slot_prec[slot] = prec;
     ++issued;
     return LYM_OK;
   };
@@ -1081,13 +1342,22 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
       set_error("fast CG-M did not reach its stop state (internal error)");
       return LYM_ERR_HIP;
     }
-    if (s4[1] >= nrhs && prec == P64 && lowprec == P32) {
-      hipLaunchKernelGGL(k_fast_switch<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
-                         (const double2*)P.ring64.data, ring_stride64, nslot64, (float2*)P.ring32.data,
-                         ring_stride32, nslot32, (const double2*)P.plive64.data, (float2*)P.plive32.data,
-                         kFastBlock32);
-      LYM_HIP(hipGetLastError());
-      prec = P32;
+    if (s4[1] >= nrhs && slot_prec[slot] == prec && prec != lowprec) {
+      if (prec == P64) {
+        hipLaunchKernelGGL(k_fast_switch<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, P.ctl, n,
+                           (const double2*)P.ring64.data, ring_stride64, nslot64, (float2*)P.ring32.data,
+                           ring_stride32, nslot32, (const double2*)P.plive64.data, (float2*)P.plive32.data, mlo);
+        LYM_HIP(hipGetLastError());
+        prec = P32;
+      } else {
+        int e = by_ncomp([&](auto nc) {
+          hipLaunchKernelGGL(k_fast_switch16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, P.st, P.ctl,
+                             V, (const float2*)P.ring32.data, nslot32, P.ring16, nslot16,
+                             (const float2*)P.plive32.data, P.plive16, mlo);
+        });
+        if (e) return e;
+        prec = P16;
+      }
     }
     if ((rc = issue(slot))) return rc;
   }
@@ -1235,6 +1505,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   const double tol_iter = 0.5 * tol;  // iterate below the target; the true residual decides
   init_cgm_state(h, 1, &sigma, &tol_iter, max_iter);
   h.phase = PH_KP;
+  h.precision_phase = pl == P16 ? 2 : 1;
   // a reliable update after every 100x drop of the iterated residual (delta =
   // 0.01).  Measured against 10x (delta = 0.1) on the 28-source batches: the
   // same iteration counts (729 / 255 against 732 / 257 at 24^3x48 / 16^3x32)
@@ -1361,6 +1632,12 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
 
 static void fill_stats(lym_solve_stats* s, const SolverState& h, int nshift) {
   if (!s) return;
+  for (int k = 0; k < 64; ++k) {
+    s->initial_rel_resid[k] = 0.0;
+    s->refine_cycles[k] = 0;
+  }
+  s->refine_iterations = 0;
+  s->precision_phase = h.precision_phase;
   s->status = h.status;
   s->iterations = h.iter;
   s->nshift = nshift;
@@ -1369,15 +1646,23 @@ static void fill_stats(lym_solve_stats* s, const SolverState& h, int nshift) {
 }
 
 // FP64 true residual of every (rhs, shift) and defect correction where one
-// misses the target.  xs[r * nshift + k]; rel[r * nshift + k] receives the
-// final true relative residuals; returns the number of correction cycles.
+// misses the target.  xs[r * nshift + k]; per (rhs, shift) entry r * nshift + k:
+// rel0 = true relative residual before correction, rel = after, cycles =
+// correction solves; iters[r] = iterations of RHS r's correction solves.
+struct RefineOut {
+  std::vector<double> rel0, rel;
+  std::vector<int> cycles;
+  std::vector<int> iters;  // per RHS
+};
 static int verify_and_refine(const OpDesc& d, const Parts& P, int prec, int nrhs, int nshift, const double* shifts,
                              const void* const* bs, void* const* xs, double tol, int max_iter,
-                             const std::vector<SolverState>& hs, hipStream_t st, std::vector<double>& rel,
-                             int* restarts) {
+                             const std::vector<SolverState>& hs, hipStream_t st, RefineOut& ro) {
   const int64_t n = field_elems(d);
   const int max_refine = 12;
-  rel.assign((size_t)nrhs * nshift, 0.0);
+  ro.rel.assign((size_t)nrhs * nshift, 0.0);
+  ro.rel0.assign((size_t)nrhs * nshift, 0.0);
+  ro.cycles.assign((size_t)nrhs * nshift, 0);
+  ro.iters.assign(nrhs, 0);
   for (int r = 0; r < nrhs; ++r) {
     // a right-hand side whose solve broke down is reported, not refined
     const bool breakdown = hs[r].status == ST_BREAKDOWN;
@@ -1390,6 +1675,7 @@ static int verify_and_refine(const OpDesc& d, const Parts& P, int prec, int nrhs
       double r2 = 0;
       int rc = true_residual(d, P, bf, xk, shifts[k], P.r64, &r2, st);
       if (rc) return rc;
+      ro.rel0[(size_t)r * nshift + k] = std::sqrt(r2);
       int cycles = 0;
       while (std::sqrt(r2) > tol && cycles < max_refine && !breakdown) {
         const double relnow = std::sqrt(r2);
@@ -1416,12 +1702,13 @@ static int verify_and_refine(const OpDesc& d, const Parts& P, int prec, int nrhs
                              (double2*)xk_ptr);
           LYM_HIP(hipGetLastError());
         }
-        ++*restarts;
+        ro.iters[r] += hi.iter;
         ++cycles;
         rc = true_residual(d, P, bf, xk, shifts[k], P.r64, &r2, st);
         if (rc) return rc;
       }
-      rel[(size_t)r * nshift + k] = std::sqrt(r2);
+      ro.cycles[(size_t)r * nshift + k] = cycles;
+      ro.rel[(size_t)r * nshift + k] = std::sqrt(r2);
     }
   }
   return LYM_OK;
@@ -1452,8 +1739,8 @@ int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const dou
       set_error("CG-M needs non-negative shifts");
       return LYM_ERR_ARG;
     }
-  if (prec == P16 && (nshift != 1 || d.adj)) {
-    set_error("half-precision storage is built for one-shift CG in the fundamental representation");
+  if (prec == P16 && d.adj) {
+    set_error("half-precision storage is built for the fundamental representation");
     return LYM_ERR_UNSUPPORTED;
   }
   const int nbatch = prec == P64 ? 1 : nrhs;  // plain FP64 runs one RHS at a time
@@ -1466,7 +1753,8 @@ int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const dou
   char* base = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
   carve(d, prec, nshift, nbatch, base, &P);
   // iterate slightly below the target so the recomputed true residual meets it
-  std::vector<double> tol_rel(nshift, 0.5 * tol);
+  const CgmTuning tun = cgm_tuning();
+  std::vector<double> tol_rel(nshift, tun.iter_factor * tol);
   std::vector<SolverState> hs;
   int rc = 0;
   if (prec == P64) {
@@ -1480,32 +1768,41 @@ int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const dou
       if (rc) return rc;
     }
   } else {
-    // mixed: FP64 Krylov vectors until every base residual is below 1e-4 of
-    // its |b|, FP32 after; shift solutions and directions accumulate in FP64
+    // mixed: FP64 Krylov vectors until every base residual is below
+    // switch32 |b|, complex64 after (and 16-bit storage below switch16 |b|
+    // for LYM_PREC_F16); shift solutions and directions accumulate in FP64
     // through deferred block updates.  FP64 true residuals are checked below.
-    // One shift: complex64 CG with FP64 reliable updates instead.
+    // One shift: complex64 or 16-bit CG with FP64 reliable updates instead.
     if (nshift == 1)
       rc = cg_reliable(d, P, prec == P16 ? P16 : P32, bs, xs, nrhs, shifts[0], tol, max_iter, st, hs);
     else
-      rc = cgm_fast(d, P, bs, xs, nrhs, nshift, shifts, tol_rel.data(), max_iter, 1e-4, P32, st, hs);
+      rc = cgm_fast(d, P, bs, xs, nrhs, nshift, shifts, tol_rel.data(), max_iter, tun, prec == P16 ? P16 : P32, st,
+                    hs);
     if (rc) return rc;
   }
-  std::vector<double> rel;
-  int restarts = 0;
-  rc = verify_and_refine(d, P, prec, nrhs, nshift, shifts, bs, xs, tol, max_iter, hs, st, rel, &restarts);
+  RefineOut ro;
+  rc = verify_and_refine(d, P, prec, nrhs, nshift, shifts, bs, xs, tol, max_iter, hs, st, ro);
   if (rc) return rc;
   int worst = LYM_OK;
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   for (int r = 0; r < nrhs; ++r) {
     int status = hs[r].status;
-    for (int k = 0; k < nshift; ++k)
-      if (rel[(size_t)r * nshift + k] > tol && status == ST_CONVERGED) status = ST_MAXITER;
+    int cycles = 0;
+    for (int k = 0; k < nshift; ++k) {
+      if (ro.rel[(size_t)r * nshift + k] > tol && status == ST_CONVERGED) status = ST_MAXITER;
+      cycles += ro.cycles[(size_t)r * nshift + k];
+    }
     if (stats) {
       lym_solve_stats* s = &stats[r];
       fill_stats(s, hs[r], nshift);
       s->status = status;
-      s->restarts = restarts + hs[r].n_ru;  // refinement cycles + reliable updates
-      for (int k = 0; k < nshift && k < 64; ++k) s->true_rel_resid[k] = rel[(size_t)r * nshift + k];
+      s->restarts = cycles + hs[r].n_ru;  // defect-correction solves + reliable updates of this RHS
+      for (int k = 0; k < nshift && k < 64; ++k) {
+        s->true_rel_resid[k] = ro.rel[(size_t)r * nshift + k];
+        s->initial_rel_resid[k] = ro.rel0[(size_t)r * nshift + k];
+        s->refine_cycles[k] = ro.cycles[(size_t)r * nshift + k];
+      }
+      s->refine_iterations = ro.iters[r];
       s->seconds = secs;
     }
     const int code = status == ST_CONVERGED ? LYM_OK : (status == ST_BREAKDOWN ? LYM_BREAKDOWN : LYM_NOT_CONVERGED);

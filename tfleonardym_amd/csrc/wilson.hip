@@ -166,7 +166,9 @@ struct EpiCgmK2 {
 // Fast CG-M step 2: residuals live in a ring of nslot fields; iteration k reads
 // r^k from slot k % nslot and writes r^{k+1} to slot (k+1) % nslot, so the
 // last block's worth of residuals stays available for the deferred shift
-// updates.  r^{k+1} = r^k - alpha (Q t + s0 p); reduce |r^{k+1}|^2.
+// updates.  r^{k+1} = r^k - alpha (Q t + s0 p); reduce |r^{k+1}|^2.  16-bit
+// rings keep one norm per site and slot: slot q of RHS i at
+// norm + (i * nslot + q) * V.
 template <int P, int NCOMP>
 struct EpiCgmK2F {
   using R = typename Store<P>::R;
@@ -174,6 +176,7 @@ struct EpiCgmK2F {
   static constexpr int NRED = 1;
   SpinorView<P> p;
   E* ring;
+  float* ring_norm;
   int64_t slot_elems;
   int nslot;
   SolverState* st;
@@ -181,6 +184,7 @@ struct EpiCgmK2F {
   EpiCgmK2F(const EpiArgs& a, int64_t V)
       : p(field_view<P>(a.aux0, V)),
         ring((E*)a.ring.data),
+        ring_norm(a.ring.norm),
         slot_elems(a.slot_elems),
         nslot(a.nslot),
         st(a.st),
@@ -190,6 +194,7 @@ struct EpiCgmK2F {
     EpiCgmK2F e = *this;
     e.p = rhs_view(p, rhs, stride);
     e.ring = ring + rhs * ring_stride;
+    if (ring_norm) e.ring_norm = ring_norm + (int64_t)rhs * nslot * p.V;
     e.st = st + rhs;
     return e;
   }
@@ -199,8 +204,8 @@ struct EpiCgmK2F {
                         double* red) const {
     const int k = st->iter;
     const int si = k % nslot, so = (k + 1) % nslot;
-    const SpinorView<P> rin{ring + si * slot_elems, nullptr, lat.V};
-    const SpinorView<P> rout{ring + so * slot_elems, nullptr, lat.V};
+    const SpinorView<P> rin{ring + si * slot_elems, ring_norm ? ring_norm + si * lat.V : nullptr, lat.V};
+    const SpinorView<P> rout{ring + so * slot_elems, ring_norm ? ring_norm + so * lat.V : nullptr, lat.V};
     cpx<R> c[NCOMP], res[NCOMP], pv[NCOMP], rv[NCOMP];
     in.template load<NCOMP>(s, c);
     p.template load<NCOMP>(s, pv);
