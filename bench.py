@@ -1,15 +1,22 @@
 """Benchmark of the fermion-matrix hot path on one MI355X.
 
-Workload (default): 4D 16^3 x 32 SU(3) Wilson D^dag D, kappa = 0.13, links
-U = exp(i 0.3 H) (seeded), one Gaussian source, multishift CG over 12 shifts
-log-spaced in [1e-4, 10], every shift to true relative residual 1e-10
-(recomputed in FP64).  One step = one complete multishift solve.
+Workload (default, the largest single-GPU configuration): 4D 32^3 x 64 SU(3)
+Wilson D^dag D, kappa = 0.1340, links U = exp(i 0.2 H) (seeded), one
+Gaussian source, multishift CG over 24 shifts log-spaced in [1e-5, 20] (a
+rational (D^dag D)^{-1/2}-style shift list), every shift to true relative
+residual 1e-10 recomputed in FP64; Krylov vectors FP64, then complex64, then
+16-bit storage (int16 + per-site norm) with FP32 arithmetic, shift solutions
+and recurrences in FP64.  One step = one complete multishift solve.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|double]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c2|c3|...]
+                    [--precision half|mixed|double]
 
-Prints one JSON line (metric, value = solves/s, ms_per_step, p50/p99 solve
-latency, Wilson-kernel HBM GB/s, roofline of the dominant kernel, CPU
-baseline).
+Prints one JSON line: metric, value = solves/s, ms_per_step, p50/p99 solve
+latency, the plain FP64 solver's rate on the same solve, every shift's FP64
+true residual, the roofline of the solve's dominant kernel (HIP-event
+durations of its launches inside the timed solves, algorithmic bytes), and
+the CPU baseline (torch-CPU complex128 restatement of the reference's op
+sequence, 4 threads and all cores).
 """
 import argparse
 import json
@@ -387,6 +394,17 @@ def measured_force_traffic(geometry):
     return None
 
 
+def calib_copy():
+    """Three copies of a 256 MiB float32 tensor on the device: the kernel of
+    known size that calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    c = torch.empty_like(a)
+    for _ in range(3):
+        c.copy_(a)
+    torch.cuda.synchronize()
+
+
 def run_dslash(args):
     """Only D (for kernel profiles) at --dslash-geometry, then a 256 MiB
     device copy (the calibration kernel of the FETCH_SIZE / WRITE_SIZE
@@ -394,11 +412,7 @@ def run_dslash(args):
     import torch
     geometry = tuple(int(v) for v in args.dslash_geometry.split("x"))
     res = dslash_c3(args.dslash_reps, geometry, autotune=not args.no_autotune)
-    a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-    c = torch.empty_like(a)
-    for _ in range(3):
-        c.copy_(a)
-    torch.cuda.synchronize()
+    calib_copy()
     f32 = res["f32"]
     print(json.dumps({"metric": f"Wilson D GB/s ({args.dslash_geometry} SU(3), compulsory bytes)", "value": f32["gbs"],
                       "unit": "GB/s", "n_gpus": 1, "steps": args.dslash_reps, "warmup": 5,
@@ -407,22 +421,111 @@ def run_dslash(args):
                       "config": {"workload": f"wilson_d_{args.dslash_geometry}_su3"}, "dslash": res}))
 
 
+def cpu_info():
+    """Host CPU of this box: logical CPUs, the CPUs this process may use, model."""
+    import platform
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"nproc": os.cpu_count(), "available": avail, "model": model}
+
+
+def cpu_baseline_torch(U, b, geometry, kappa, shifts, gpu_iterations, iters):
+    """The reference's op sequence restated in torch-CPU complex128
+    (oracle/torch_cpu.py: project -> gather -> colour einsum -> rebuild ->
+    add -> g5, dirac.py:458-501) driving multishift CG iterations with every
+    shift active; timed with 4 intra-op threads (the reference's
+    TfLeonardYMNDim.py:16) and with every CPU this process may use, on
+    `iters` iterations after one warm-up iteration, extrapolated to the GPU
+    solve's iteration count.  TensorFlow is not installed, so the reference
+    itself cannot be timed."""
+    import torch
+    from oracle import torch_cpu as otc
+    info = cpu_info()
+    threads_all = max(1, min(int(os.environ.get("OMP_NUM_THREADS", info["available"])), info["available"]))
+    t0 = time.time()
+    op = otc.TorchWilson(U, geometry, kappa, True)
+    bt = torch.from_numpy(b)
+    res = {}
+    for nt in sorted({4, threads_all}):
+        torch.set_num_threads(nt)
+        otc.cgm_iterations(op, bt, shifts, 1)
+        tc = time.perf_counter()
+        otc.cgm_iterations(op, bt, shifts, iters)
+        per_it = (time.perf_counter() - tc) / iters
+        res[nt] = per_it
+        print(f"[bench] cpu baseline ({nt} threads): {per_it:.2f} s per CG-M iteration", file=sys.stderr, flush=True)
+    print(f"[bench] cpu baseline total {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    best_nt = min(res, key=res.get)
+    sample = (f"{iters} multishift CG iterations ({len(shifts)} shifts, all active) at {'x'.join(map(str, geometry))} "
+              f"by a torch-CPU complex128 restatement of the reference's op sequence (oracle/torch_cpu.py; "
+              f"TensorFlow unavailable offline), after one warm-up iteration, extrapolated to the "
+              f"{gpu_iterations} iterations of the GPU solve")
+    return {"value": 1.0 / (res[best_nt] * gpu_iterations), "unit": "solves/s", "cores": best_nt, "kind": "port",
+            "sample": sample,
+            "per_threads": {str(nt): {"s_per_iteration": res[nt], "solves_per_s": 1.0 / (res[nt] * gpu_iterations)}
+                            for nt in res},
+            "cpu": info}
+
+
+# Kernel classes of lym_kernel_timing and the kernels they are in a rocprof trace
+KERNEL_OF_CLASS = {
+    "shift_update_f64": "k_fast_k3", "shift_update_f32": "k_fast_k3s<1>", "shift_update_f16": "k_fast_k3s<2>",
+    "wilson_cgm_k1_f64": "k_wilson_tile<..., 0, EpiCgmK1<0, 12>, ...>",
+    "wilson_cgm_k2f_f64": "k_wilson_tile<..., 0, EpiCgmK2F<0, 12>, ...>",
+    "wilson_cgm_k1_f32": "k_wilson_tile<..., 1, EpiCgmK1<1, 12>, ...>",
+    "wilson_cgm_k2f_f32": "k_wilson_tile<..., 1, EpiCgmK2F<1, 12>, ...>",
+    "wilson_cgm_k1_f16": "k_wilson_tile<..., 2, EpiCgmK1<2, 12>, ...>",
+    "wilson_cgm_k2f_f16": "k_wilson_tile<..., 2, EpiCgmK2F<2, 12>, ...>",
+    "direction_update_f64": "k_fast_kp<0>", "direction_update_f32": "k_fast_kp<1>",
+    "direction_update_f16": "k_fast_kp16<12>", "finalize": "k_finalize",
+}
+
+
+def measured_class_traffic(workload, cls):
+    """HBM bytes per launch of kernel class `cls` in `workload` from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, or None."""
+    path = os.path.join(ROOT, "profiles", "multishift_c5_traffic.md")
+    if not os.path.exists(path):
+        return None
+    import re
+    for line in open(path):
+        m = re.search(r"traffic_bytes_per_launch (\S+) (\S+) = (\d+)", line)
+        if m and m.group(1) == workload and m.group(2) == cls:
+            return int(m.group(3))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(SERVING) + ["dslash"])
+    ap.add_argument("--workload", default="c5", choices=sorted(WORKLOADS) + sorted(BATCHED) + sorted(SERVING) + ["dslash"])
     ap.add_argument("--precision", default=None, choices=["mixed", "double", "half"],
-                    help="Krylov precision (default: mixed; batched workloads: their own)")
+                    help="Krylov storage (default: half for c5, mixed for the other multishift workloads; "
+                         "batched workloads: their own)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--dslash-reps", type=int, default=50)
     ap.add_argument("--no-dslash-c3", action="store_true", help="skip the 24^3x48 dslash measurement")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched 28-source CG at 24^3x48 reported beside the default workload")
     ap.add_argument("--dslash-geometry", default="24x24x24x48", help="lattice of --workload dslash")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in Wilson tile rule")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="time the solves without HIP events around their kernels (HIP-graph replay)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="end with 256 MiB device copies (the calibration kernel of rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--baseline-steps", type=int, default=2,
                     help="timed solves of the plain FP64 CG-M for the speed-up figure (0: skip)")
     args = ap.parse_args()
@@ -436,13 +539,12 @@ def main():
     if args.workload == "dslash":
         return run_dslash(args)
     if args.precision is None:
-        args.precision = "mixed"
+        args.precision = "half" if args.workload == "c5" else "mixed"
     if args.workload in SERVING:
         return run_serving(args)
 
     import torch
     from tfleonardym_amd import _lib, dirac, dirac_solver, fields, lattice
-    import ctypes
 
     dev = _lib.require_device()
     W = WORKLOADS[args.workload]
@@ -465,13 +567,16 @@ def main():
     solver = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10,
                                             precision_mode=args.precision, verbose=False)
 
-    # ---- warmup + timed solves ------------------------------------------------
+    # ---- warmup + timed solves (inputs resident in HBM) ---------------------
     for i in range(args.warmup):
         solver.solve(A, bd, shifts)
         print(f"[bench] warmup {i} done: {solver.last_stats['iterations']} it, "
               f"{solver.last_stats['seconds']:.3f}s", file=sys.stderr, flush=True)
+    timing = not args.no_kernel_timing and args.precision != "double"
     lat = []
     torch.cuda.synchronize()
+    if timing:
+        _lib.kernel_timing(True)
     t_start = time.perf_counter()
     stats = None
     for i in range(args.steps):
@@ -483,10 +588,14 @@ def main():
         if stats["status"] != "converged" or max(stats["true_rel_resid"]) > 1e-10:
             print(f"[bench] solve {i} did not reach 1e-10: {stats}", file=sys.stderr)
             sys.exit(1)
-        print(f"[bench] step {i}: {lat[-1] * 1e3:.1f} ms, {stats['iterations']} it, restarts {stats['restarts']}",
-              file=sys.stderr, flush=True)
+        print(f"[bench] step {i}: {lat[-1] * 1e3:.1f} ms, {stats['iterations']} it, refinement "
+              f"{sum(stats['refine_cycles'])} solves / {stats['refine_iterations']} it", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     total = time.perf_counter() - t_start
+    kt = None
+    if timing:
+        _lib.kernel_timing(False)
+        kt = _lib.kernel_timing_read()
     solves_per_s = args.steps / total
 
     # ---- the plain FP64 multishift CG (textbook CG-M, one RHS) on the same solve
@@ -501,42 +610,55 @@ def main():
             ps.solve(A, bd, shifts)
         torch.cuda.synchronize()
         plain = args.baseline_steps / (time.perf_counter() - tb)
-        print(f"[bench] plain FP64 CG-M: {plain:.2f} solves/s, {ps.last_stats['iterations']} it", file=sys.stderr,
+        print(f"[bench] plain FP64 CG-M: {plain:.3f} solves/s, {ps.last_stats['iterations']} it", file=sys.stderr,
               flush=True)
+        del ps
+    del xs
+
+    # ---- roofline: the solve's dominant kernel class ------------------------
+    roofline = None
+    kernels = None
+    if kt:
+        kernels = {k: dict(v, avg_us=v["us"] / max(1, v["launches"]),
+                           gbs=(v["bytes"] / (v["us"] * 1e-6) / 1e9) if v["us"] > 0 and v["bytes"] > 0 else None,
+                           share=v["us"] / (total * 1e6))
+                   for k, v in kt.items()}
+        dom = max((k for k in kt if kt[k]["bytes"] > 0), key=lambda k: kt[k]["us"])
+        d = kt[dom]
+        gbs = d["bytes"] / (d["us"] * 1e-6) / 1e9
+        roofline = {"kernel": f"{dom} ({KERNEL_OF_CLASS.get(dom, dom)})", "bound": "hbm", "achieved": gbs,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                    "traffic": measured_class_traffic(W["name"], dom),
+                    "bytes_per_launch": d["bytes"] / d["launches"], "avg_us": d["us"] / d["launches"],
+                    "launches": d["launches"], "share_of_solve": d["us"] / (total * 1e6),
+                    "timing": "HIP events on the solve stream around every launch of the class, timed solves"}
+        print(f"[bench] dominant kernel {dom}: {d['launches']} launches, {d['us'] / d['launches']:.1f} us avg, "
+              f"{gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of {HBM_PEAK_GBS:.0f})", file=sys.stderr, flush=True)
+
+    if args.calib_copy:
+        calib_copy()
 
     # ---- Wilson kernel bandwidth (HIP events on the launch stream) ----------
     res = time_dslash(op, bd, args.dslash_reps)
     for name in ("f64", "f32"):
         print(f"[bench] wilson {name}: {res[name]['us']:.2f} us/launch, {res[name]['gbs']:.0f} GB/s", file=sys.stderr,
               flush=True)
+    del op, A, solver
+    torch.cuda.empty_cache()
     res_c3 = None
     if not args.no_dslash_c3:
         res_c3 = dslash_c3(args.dslash_reps, autotune=not args.no_autotune)
 
     # ---- the batched 28-source CG at 24^3x48 (16-bit storage) beside it ---
     res_b = None
-    if not args.no_batched and args.workload == "c2":
+    if not args.no_batched:
         res_b = batched_result("c3b", None, 1, 1, 2, autotune=args.no_dslash_c3 and not args.no_autotune)
 
-    # ---- CPU baseline: the oracle (NumPy restatement of the reference op
-    # sequence), a bounded number of FP64 CG-M iterations, extrapolated ------
+    # ---- CPU baseline -----------------------------------------------------
     cpu = None
     if not args.no_cpu:
-        from oracle import solvers as osv
-        from oracle import wilson as ow
-        oop = ow.WilsonOracle(U, geometry, W["kappa"], True, "fundamental")
-        tc = time.perf_counter()
-        osv.cgm(oop.apply_square, b, shifts, tol=1e-10, max_iter=args.cpu_iters)
-        per_it = (time.perf_counter() - tc) / args.cpu_iters
-        its = stats["iterations"]
-        cpu = {"value": 1.0 / (per_it * its), "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_iters} CG-M iterations (12 shifts) of this solve by the NumPy complex128 "
-                         f"oracle following the reference op sequence, {per_it:.2f} s/iteration, extrapolated "
-                         f"to the {its} iterations of the GPU solve"}
-        print(f"[bench] cpu baseline {per_it:.2f} s/iteration", file=sys.stderr, flush=True)
+        cpu = cpu_baseline_torch(U, b, geometry, W["kappa"], shifts, stats["iterations"], args.cpu_iters)
 
-    f32 = res["f32"]
-    desc = op.desc
     out = {
         "metric": "multishift-CG solves/s at 1e-10 true resid",
         "value": solves_per_s,
@@ -548,27 +670,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "dtype": {"half": "f16 storage (int16 + per-site norm) + f32 + f64", "mixed": "f32+f64",
+                  "double": "f64"}[args.precision],
         "data": "synthetic (seeded near-identity SU(3) links, Gaussian source)",
         "config": {"workload": W["name"], "geometry": list(geometry), "kappa": W["kappa"], "shifts": len(shifts),
                    "shift_range": [W["lo"], W["hi"]], "precision": args.precision, "tolerance": 1e-10},
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
         "iterations": stats["iterations"],
+        "precision_phase_reached": stats["precision_phase"],
+        "refinement": {"solves": sum(stats["refine_cycles"]), "iterations": stats["refine_iterations"]},
         "plain_fp64_solves_per_s": plain,
         "speedup_vs_plain_fp64": (solves_per_s / plain) if plain else None,
         "max_true_rel_resid": max(stats["true_rel_resid"]),
         "true_rel_resid_per_shift": [float(v) for v in stats["true_rel_resid"]],
-        "dslash_gbs": {"f64": res["f64"]["gbs"], "f32": f32["gbs"]},
-        "roofline": {"kernel": "Wilson D complex64, one RHS (k_wilson_tile)", "bound": "hbm",
-                     "achieved": f32["gbs"], "link_format": "su-rebuilt" if desc.link_format == _lib.LINKS_SU
-                     else "full",
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": f32["gbs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(geometry, "f32"),
-                     "bytes_per_site": f32["bytes_per_site"], "sites": V, "avg_us": f32["us"]},
+        "roofline": roofline,
+        "kernels": kernels,
+        "dslash_gbs": {"f64": res["f64"]["gbs"], "f32": res["f32"]["gbs"]},
+        "dslash": res,
         "dslash_24c3x48": res_c3,
         "batched_24c3x48": res_b,
-        "wilson_tiles": {"16c3x32": tiles or {"f32": op.tile_info("f32"), "f64": op.tile_info("f64")},
-                         "24c3x48": res_c3.get("tiles") if res_c3 else None},
+        "wilson_tiles": tiles,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

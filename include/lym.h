@@ -205,6 +205,28 @@ int lym_tune_wilson(const lym_wilson* op, int prec, int reps, const void* in, vo
                     void* stream);
 int lym_tile_info(const lym_wilson* op, int prec, lym_tile_choice* out);
 
+/* ---- kernel timing (no reference counterpart; measurement only) --------
+ * lym_kernel_timing(1) clears the counters and starts timing on the calling
+ * host thread: every multishift CG solve with several shifts and
+ * LYM_PREC_F32 / LYM_PREC_F16 (lym_cgm, lym_cgm_batched) then brackets each
+ * of its Wilson, direction-update, shift-update and finalizer launches with
+ * HIP events on the solve's stream (launching directly instead of replaying
+ * HIP graphs) and, when the solve ends, adds per kernel class the number of
+ * launches, the summed event durations and the summed algorithmic HBM bytes
+ * (each link row read and each vector read or written once; a shift
+ * update's bytes count its block's residuals and the shifts active in it).
+ * lym_kernel_timing(0) stops timing.  lym_kernel_timing_read copies up to
+ * `max` classes to `out` and the number of classes to *count. */
+typedef struct lym_kernel_time {
+  char name[48];
+  int32_t launches;
+  int32_t reserved;
+  double total_us;
+  double bytes;
+} lym_kernel_time;
+int lym_kernel_timing(int enable);
+int lym_kernel_timing_read(lym_kernel_time* out, int max, int* count);
+
 /* ---- solvers ---------------------------------------------------------- */
 /* Device workspace (bytes) for the solvers below. */
 size_t lym_solver_workspace_bytes(const lym_wilson* op, int prec, int nshift);

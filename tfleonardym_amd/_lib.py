@@ -58,6 +58,17 @@ class LymTileChoice(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class LymKernelTime(ctypes.Structure):
+    """lym_kernel_time: one kernel class of a timed solve."""
+    _fields_ = [
+        ("name", ctypes.c_char * 48),
+        ("launches", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("total_us", ctypes.c_double),
+        ("bytes", ctypes.c_double),
+    ]
+
+
 class LymSolveStats(ctypes.Structure):
     _fields_ = [
         ("status", ctypes.c_int32),
@@ -125,6 +136,8 @@ SIGNATURES = {
                                  ctypes.POINTER(LymSolveStats), _vp]),
     "lym_cgm": (_i32, [ctypes.POINTER(LymWilson), _i32, _i32, ctypes.POINTER(_f64), _vp, ctypes.POINTER(_vp),
                        _f64, _i32, _vp, _sz, ctypes.POINTER(LymSolveStats), _vp]),
+    "lym_kernel_timing": (_i32, [_i32]),
+    "lym_kernel_timing_read": (_i32, [ctypes.POINTER(LymKernelTime), _i32, ctypes.POINTER(_i32)]),
     "lym_cgm_batched_workspace_bytes": (_sz, [ctypes.POINTER(LymWilson), _i32, _i32, _i32]),
     "lym_cgm_batched": (_i32, [ctypes.POINTER(LymWilson), _i32, _i32, _i32, ctypes.POINTER(_f64),
                                ctypes.POINTER(_vp), ctypes.POINTER(_vp), _f64, _i32, _vp, _sz,
@@ -217,3 +230,20 @@ def reduce_scratch(device):
         t = torch.zeros(n + 512, dtype=torch.uint8, device=device)
         _scratch[key] = t
     return t
+
+
+def kernel_timing(enable):
+    """Start (clearing the counters) or stop the kernel timing of the fast
+    multishift CG on this host thread (lym_kernel_timing)."""
+    check(lib().lym_kernel_timing(1 if enable else 0), "kernel_timing")
+
+
+def kernel_timing_read():
+    """Per kernel class of the timed solves: {name: dict(launches, us,
+    bytes)} (lym_kernel_timing_read)."""
+    n = ctypes.c_int(0)
+    check(lib().lym_kernel_timing_read(None, 0, ctypes.byref(n)), "kernel_timing_read")
+    arr = (LymKernelTime * max(1, n.value))()
+    check(lib().lym_kernel_timing_read(arr, n.value, ctypes.byref(n)), "kernel_timing_read")
+    return {arr[i].name.decode(): dict(launches=int(arr[i].launches), us=float(arr[i].total_us),
+                                       bytes=float(arr[i].bytes)) for i in range(n.value)}

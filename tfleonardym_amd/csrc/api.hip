@@ -622,4 +622,14 @@ int lym_cgm_batched(const lym_wilson* op, int prec, int nrhs, int nshift, const 
                            (hipStream_t)stream);
 }
 
+int lym_kernel_timing(int enable) { return kernel_timing(enable); }
+
+int lym_kernel_timing_read(lym_kernel_time* out, int max, int* count) {
+  if ((!out && max > 0) || max < 0) {
+    set_error("lym_kernel_timing_read: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  return kernel_timing_read(out, max, count);
+}
+
 }  // extern "C"

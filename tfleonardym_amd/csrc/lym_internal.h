@@ -155,6 +155,10 @@ int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* o
 // out_dev[0] = max |stored - rebuilt| over the last rows of every link
 int link_deviation(const OpDesc& d, int prec, double* out_dev, hipStream_t st);
 
+// ---- kernel timing of the fast multishift CG (solvers.hip) ------------------
+int kernel_timing(int enable);
+int kernel_timing_read(lym_kernel_time* out, int max, int* count);
+
 // ---- measurements (measure.hip) ---------------------------------------------
 constexpr int kSliceMaxVec = 32;
 struct SliceVecs {

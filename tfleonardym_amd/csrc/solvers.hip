@@ -333,84 +333,62 @@ __global__ __launch_bounds__(kBlock) void k_fast_kp16(const SolverState* st, int
 // steps j < n_upd[s]:  x_s += a_j p_s ;  p_s = z_j r^{k0+j+1} + b_j p_s
 // with x_s, p_s in FP64.  One thread per element (component plane c, site);
 // the block's residuals r^{k0+1..k0+nb} at that element are loaded once into
-// registers, then the active shifts stream their x_s, p_s through them.  Per
-// block the kernel moves nb residuals plus 4 vectors per active shift, so
-// the shift traffic per iteration falls with the block length MB, at the
-// cost of 2 MB (float) or 4 MB (double) residual registers.
-//   k_fast_k3 (FP64 ring, MB = 16): 64 VGPRs, occupancy 8; the shifts are
-//     updated one after the other, their loads kept in flight by occupancy.
-//   k_fast_k3s (complex64 / 16-bit ring decoded to float, MB = 32): about
-//     100 VGPRs, occupancy 4; the next shift's pair is loaded while the
-//     current one is updated.
+// registers (complex128, or complex64 / 16-bit decoded to float, a 16-bit
+// ring's site norm multiplying the integers), then the active shifts stream
+// their x_s, p_s through them, the next shift's pair loaded while the current
+// one is updated.  Per block the kernel moves nb residuals plus 4 vectors per
+// active shift, so the shift traffic per iteration falls with the block
+// length MB (16 for the FP64 ring, 64 VGPRs; 32 for the float ring, 64
+// VGPRs).  The workgroup first copies the active shifts' step coefficients
+// (a, z, b) and the list of active shifts into LDS: read per wave from the
+// scalar cache they did not fit it (24 shifts x 32 steps x 24 B = 18 KiB)
+// and the kernel stalled on scalar loads at 1.3-2.4 TB/s.
 // Work item m covers site 64 * (m / 64 / ncomp) + m % 64 of plane
 // (m / 64) % ncomp: consecutive waves take the planes of the same 64 sites,
 // so the per-site norms of a 16-bit ring are re-read from cache.
-// (Measured on MI355X at 32^3x64 with 24 shifts: complex128 blocks of 32 or
-// complex64 blocks of 64, which need 128 residual registers, ran at
-// occupancy 1-3 and gave no gain; 16 and 32 are the fastest.)
-#define LYM_K3_ITEM(m, ncomp, V, c, site, i)              \
-  const int64_t grp_ = (m) >> 6;                          \
-  const int c = (int)(grp_ % (ncomp));                    \
-  const int64_t site = (grp_ / (ncomp)) * 64 + ((m) & 63); \
-  if (site >= (V)) continue;                              \
-  const int64_t i = (int64_t)c * (V) + site;
-
 constexpr int kK3Block64 = 16;
 constexpr int kK3BlockLo = 32;
 
+template <int PR>
 __global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int ncomp, int64_t V, Field ring,
                                                     int64_t ring_stride, int nslot, const Field* xs, const Field* ps,
                                                     int nshift) {
-  constexpr int MB = kK3Block64;
-  const int rhs = blockIdx.y;
-  st += rhs;
-  if (st->phase != PH_K3) return;
-  const double2* rg = (const double2*)ring.data + rhs * ring_stride;
-  xs += rhs * nshift;
-  ps += rhs * nshift;
-  const int nb = st->blk_j;
-  const int k0 = st->iter - nb;
-  const int64_t n = (int64_t)ncomp * V;
-  const int64_t nwork = ((V + 63) / 64) * ncomp * 64;
-  LYM_GRID_STRIDE(m, nwork) {
-    LYM_K3_ITEM(m, ncomp, V, c, site, i)
-    double2 r[MB];
-#pragma unroll
-    for (int j = 0; j < MB; ++j) {
-      r[j] = make_double2(0.0, 0.0);
-      if (j < nb) r[j] = rg[(int64_t)((k0 + j + 1) % nslot) * n + i];
-    }
-    for (int k = 0; k < nshift; ++k) {
-      const int mk = st->n_upd[k];
-      if (mk == 0) continue;
-      double2* xp = (double2*)xs[k].data;
-      double2* pp = (double2*)ps[k].data;
-      double2 x = xp[i], p = pp[i];
-#pragma unroll
-      for (int j = 0; j < MB; ++j) {
-        if (j < mk) {
-          const double a = st->h_alpha[k][j], z = st->h_zeta[k][j], b = st->h_beta[k][j];
-          x.x = fmad(a, p.x, x.x);
-          x.y = fmad(a, p.y, x.y);
-          p.x = fmad(z, r[j].x, b * p.x);
-          p.y = fmad(z, r[j].y, b * p.y);
-        }
-      }
-      xp[i] = x;
-      pp[i] = p;
-    }
-  }
-}
-
-template <int PR>
-__global__ __launch_bounds__(kBlock) void k_fast_k3s(const SolverState* st, int ncomp, int64_t V, Field ring,
-                                                     int64_t ring_stride, int nslot, const Field* xs, const Field* ps,
-                                                     int nshift) {
-  constexpr int MB = kK3BlockLo;
+  constexpr int MB = PR == P64 ? kK3Block64 : kK3BlockLo;
+  using RR = std::conditional_t<PR == P64, double, float>;
   using E = Elem<PR>;
+  // LDS (dynamic, k3_lds_bytes): coefficients [active shift][step] and the
+  // active shifts' indices and step counts
+  extern __shared__ __align__(16) unsigned char k3_smem[];
+  double(*sa)[MB] = reinterpret_cast<double(*)[MB]>(k3_smem);
+  double(*sz)[MB] = sa + nshift;
+  double(*sb)[MB] = sz + nshift;
+  int* sk = reinterpret_cast<int*>(sb + nshift);
+  int* sm = sk + nshift;
+  __shared__ int snact;
   const int rhs = blockIdx.y;
   st += rhs;
   if (st->phase != PH_K3) return;
+  if (threadIdx.x == 0) {
+    int na = 0;
+    for (int k = 0; k < nshift; ++k)
+      if (st->n_upd[k] > 0) {
+        sk[na] = k;
+        sm[na] = st->n_upd[k];
+        ++na;
+      }
+    snact = na;
+  }
+  __syncthreads();
+  const int nact = snact;
+  for (int t = threadIdx.x; t < nact * MB; t += blockDim.x) {
+    const int q = t / MB, j = t - q * MB;
+    const int k = sk[q];
+    sa[q][j] = st->h_alpha[k][j];
+    sz[q][j] = st->h_zeta[k][j];
+    sb[q][j] = st->h_beta[k][j];
+  }
+  __syncthreads();
+  if (nact == 0) return;
   const E* rg = (const E*)ring.data + rhs * ring_stride;
   const float* rn = ring.norm ? ring.norm + (int64_t)rhs * nslot * V : nullptr;
   xs += rhs * nshift;
@@ -419,16 +397,17 @@ __global__ __launch_bounds__(kBlock) void k_fast_k3s(const SolverState* st, int 
   const int k0 = st->iter - nb;
   const int64_t n = (int64_t)ncomp * V;
   const int64_t nwork = ((V + 63) / 64) * ncomp * 64;
-  int kfirst = 0;
-  while (kfirst < nshift && st->n_upd[kfirst] == 0) ++kfirst;
-  if (kfirst >= nshift) return;
   LYM_GRID_STRIDE(m, nwork) {
-    LYM_K3_ITEM(m, ncomp, V, c, site, i)
-    float rr[MB], ri[MB];
+    const int64_t grp = m >> 6;
+    const int c = (int)(grp % ncomp);
+    const int64_t site = (grp / ncomp) * 64 + (m & 63);
+    if (site >= V) continue;
+    const int64_t i = (int64_t)c * V + site;
+    RR rr[MB], ri[MB];
 #pragma unroll
     for (int j = 0; j < MB; ++j) {
-      rr[j] = 0.f;
-      ri[j] = 0.f;
+      rr[j] = RR(0);
+      ri[j] = RR(0);
       if (j < nb) {
         const int q = (k0 + j + 1) % nslot;
         const E e = rg[(int64_t)q * n + i];
@@ -442,29 +421,28 @@ __global__ __launch_bounds__(kBlock) void k_fast_k3s(const SolverState* st, int 
         }
       }
     }
-    int k = kfirst;
+    int k = sk[0];
     double2 x = ((const double2*)xs[k].data)[i];
     double2 p = ((const double2*)ps[k].data)[i];
-    while (k < nshift) {
-      int kn = k + 1;
-      while (kn < nshift && st->n_upd[kn] == 0) ++kn;
+    for (int qa = 0; qa < nact; ++qa) {
       double2 xn = x, pn = p;
-      if (kn < nshift) {
+      int kn = -1;
+      if (qa + 1 < nact) {  // the next shift's pair is in flight during this update
+        kn = sk[qa + 1];
         xn = ((const double2*)xs[kn].data)[i];
         pn = ((const double2*)ps[kn].data)[i];
       }
-      // converted per use: otherwise the compiler hoists the conversions out
-      // of the shift loop and keeps 4 MB doubles live instead of 2 MB floats
+      if constexpr (PR != P64) {
+        // converted per use: otherwise the compiler hoists the conversions
+        // out of the shift loop and keeps 4 MB doubles live instead of 2 MB floats
 #pragma unroll
-      for (int j = 0; j < MB; ++j) asm volatile("" : "+v"(rr[j]), "+v"(ri[j]));
-      const int mk = st->n_upd[k];
-      const double* ha = st->h_alpha[k];
-      const double* hz = st->h_zeta[k];
-      const double* hb = st->h_beta[k];
+        for (int j = 0; j < MB; ++j) asm volatile("" : "+v"(rr[j]), "+v"(ri[j]));
+      }
+      const int mk = sm[qa];
 #pragma unroll
       for (int j = 0; j < MB; ++j) {
         if (j < mk) {
-          const double a = ha[j], z = hz[j], b = hb[j];
+          const double a = sa[qa][j], z = sz[qa][j], b = sb[qa][j];
           x.x = fmad(a, p.x, x.x);
           x.y = fmad(a, p.y, x.y);
           p.x = fmad(z, (double)rr[j], b * p.x);
@@ -478,6 +456,12 @@ __global__ __launch_bounds__(kBlock) void k_fast_k3s(const SolverState* st, int 
       p = pn;
     }
   }
+}
+
+// dynamic LDS of k_fast_k3 for `nshift` shifts
+static size_t k3_lds_bytes(int prec, int nshift) {
+  const int mb = prec == P64 ? kK3Block64 : kK3BlockLo;
+  return (size_t)nshift * (3 * mb * sizeof(double) + 2 * sizeof(int));
 }
 
 // Shared state transition of a precision switch (block (0, rhs), thread 0):
@@ -752,6 +736,12 @@ static size_t field_bytes(const OpDesc& d, int prec) {
   return (size_t)n * eb;
 }
 
+// bytes per site of one spinor at storage precision prec
+static double vec_bytes_site(const OpDesc& d, int prec) {
+  const double ws = prec == P64 ? 8.0 : prec == P32 ? 4.0 : 2.0;
+  return d.ncomp * 2.0 * ws + (prec == P16 ? 4.0 : 0.0);
+}
+
 static Field take_field(Carver& c, const OpDesc& d, int prec) {
   Field f;
   f.data = c.take(field_bytes(d, prec));
@@ -973,9 +963,10 @@ struct GraphCache {
 };
 static thread_local GraphCache g_graphs;
 
+static bool kt_on();
 static bool graphs_enabled() {
   const char* v = std::getenv("LYM_NO_GRAPH");
-  return !(v && v[0] == '1');
+  return !(v && v[0] == '1') && !kt_on();
 }
 
 // a new solve: graphs launched before it have completed
@@ -1181,6 +1172,117 @@ static int cgm_core(const OpDesc& d, int prec, const Parts& P, Field b, const Fi
   return LYM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Kernel timing of the fast multishift CG (lym_kernel_timing): when enabled
+// on this host thread, every Wilson, direction-update and shift-update launch
+// of the solve is bracketed by HIP events on the solve's stream (the chunks
+// are then launched directly, not replayed from graphs), and the per-class
+// durations and algorithmic HBM bytes are added up when the solve ends.
+// Algorithmic bytes per launch: each link (the rows the kernel reads) and
+// each input / output vector once; the shift update's bytes follow from the
+// block's residuals and the shifts still active in it (conv_iter).
+// ---------------------------------------------------------------------------
+struct KClass {
+  std::string name;
+  int64_t launches = 0;
+  double us = 0.0, bytes = 0.0;
+};
+struct KRec {
+  int cls;
+  hipEvent_t e0, e1;
+  double bytes;    // known bytes, or -1: shift update of chunk `chunk`
+  int chunk;       // chunk index (shift updates)
+};
+struct KTimer {
+  bool on = false;
+  std::vector<KClass> cls;
+  std::vector<KRec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+};
+static thread_local KTimer g_kt;
+static bool kt_on() { return g_kt.on; }
+
+static int kt_class(const std::string& name) {
+  for (size_t i = 0; i < g_kt.cls.size(); ++i)
+    if (g_kt.cls[i].name == name) return (int)i;
+  g_kt.cls.push_back(KClass{name});
+  return (int)g_kt.cls.size() - 1;
+}
+static int kt_event(hipEvent_t* e) {
+  if (g_kt.used == g_kt.pool.size()) {
+    hipEvent_t x;
+    LYM_HIP(hipEventCreate(&x));
+    g_kt.pool.push_back(x);
+  }
+  *e = g_kt.pool[g_kt.used++];
+  return LYM_OK;
+}
+// time launch `f` on stream st as class `name` with `bytes` (or -1 + chunk)
+template <class F>
+static int kt_launch(hipStream_t st, const char* name, double bytes, int chunk, F&& f) {
+  if (!g_kt.on) return f();
+  KRec r{kt_class(name), nullptr, nullptr, bytes, chunk};
+  int rc = kt_event(&r.e0);
+  if (!rc) rc = kt_event(&r.e1);
+  if (rc) return rc;
+  LYM_HIP(hipEventRecord(r.e0, st));
+  rc = f();
+  LYM_HIP(hipEventRecord(r.e1, st));
+  g_kt.recs.push_back(r);
+  return rc;
+}
+// the solve has ended (stream synchronised): add durations and bytes.
+// chunk_k0[c] / chunk_nb[c]: first iteration and length of chunk c;
+// conv[s]: iteration shift s converged at (<0: never); shift_bytes(c) the
+// bytes of chunk c's shift update.
+template <class G>
+static int kt_collect(G&& shift_bytes) {
+  for (const KRec& r : g_kt.recs) {
+    float ms = 0.f;
+    LYM_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
+    KClass& c = g_kt.cls[r.cls];
+    c.launches += 1;
+    c.us += 1e3 * ms;
+    c.bytes += r.bytes >= 0 ? r.bytes : shift_bytes(r.chunk);
+  }
+  g_kt.recs.clear();
+  g_kt.used = 0;
+  return LYM_OK;
+}
+
+int kernel_timing(int enable) {
+  g_kt.on = enable != 0;
+  if (enable) {
+    g_kt.cls.clear();
+    g_kt.recs.clear();
+    g_kt.used = 0;
+  }
+  return LYM_OK;
+}
+int kernel_timing_read(lym_kernel_time* out, int max, int* count) {
+  const int n = (int)g_kt.cls.size();
+  if (count) *count = n;
+  for (int i = 0; i < n && i < max; ++i) {
+    std::memset(&out[i], 0, sizeof(out[i]));
+    std::snprintf(out[i].name, sizeof(out[i].name), "%s", g_kt.cls[i].name.c_str());
+    out[i].launches = (int32_t)g_kt.cls[i].launches;
+    out[i].total_us = g_kt.cls[i].us;
+    out[i].bytes = g_kt.cls[i].bytes;
+  }
+  return LYM_OK;
+}
+
+// algorithmic bytes per site of one Wilson launch: links (D directions x
+// rows x nc complex) + `nvec` spinors (complex numbers of 2 * ws bytes, plus
+// a 4-byte norm per site for 16-bit storage)
+static double wilson_bytes_site(const OpDesc& d, int prec, int nvec) {
+  const double wl = prec == P64 ? 8.0 : 4.0;
+  const double rows = (!d.adj && d.link_su) ? d.nc - 1 : d.nc;
+  const double links = (double)d.ndim * rows * d.nc * 2.0 * wl * (d.adj ? 2.0 : 1.0);
+  return links + nvec * vec_bytes_site(d, prec);
+}
+
 // Fast multishift CG over a batch of nrhs right-hand sides (one shift list).
 // Blocks of m iterations (CgmTuning::mblk_64 in the FP64 phase, mblk_lo after):
 //   [ p = r + beta p ; t = Q p (|t|^2,|p|^2) ; r' = r - alpha (Q t + s0 p) (|r'|^2) ] x m
@@ -1256,6 +1358,9 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   };
 
   int prec = P64;
+  // chunks issued so far (precision and length), for the kernel timing
+  std::vector<std::pair<int, int>> chunk_log;
+  auto pname = [](int p) { return p == P64 ? "f64" : p == P32 ? "f32" : "f16"; };
   auto chunk = [&](hipStream_t st) -> int {
     const Field ring = prec == P64 ? P.ring64 : prec == P32 ? P.ring32 : P.ring16;
     const Field pl = prec == P64 ? P.plive64 : prec == P32 ? P.plive32 : P.plive16;
@@ -1264,27 +1369,38 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
     const int mb = prec == P64 ? m64 : mlo;
     const int nslot = prec == P64 ? nslot64 : prec == P32 ? nslot32 : nslot16;
     const int64_t ring_stride = prec == P64 ? ring_stride64 : prec == P32 ? ring_stride32 : ring_stride16;
+    const int ci = (int)chunk_log.size();
+    chunk_log.emplace_back(prec, mb);
+    const double sites = (double)V * nrhs;
+    const std::string sfx = pname(prec);
     for (int j = 0; j < mb; ++j) {
-      if (prec == P64)
-        hipLaunchKernelGGL(k_fast_kp<P64>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const double2*)ring.data,
-                           ring_stride, nslot, (double2*)pl.data);
-      else if (prec == P32)
-        hipLaunchKernelGGL(k_fast_kp<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const float2*)ring.data,
-                           ring_stride, nslot, (float2*)pl.data);
-      else {
-        int e = by_ncomp([&](auto nc) {
-          hipLaunchKernelGGL(k_fast_kp16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, P.st, V, ring,
-                             nslot, pl);
-        });
-        if (e) return e;
-      }
-      LYM_HIP(hipGetLastError());
+      int e = kt_launch(st, ("direction_update_" + sfx).c_str(), 3.0 * vec_bytes_site(d, prec) * sites, -1, [&]() {
+        if (prec == P64)
+          hipLaunchKernelGGL(k_fast_kp<P64>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const double2*)ring.data,
+                             ring_stride, nslot, (double2*)pl.data);
+        else if (prec == P32)
+          hipLaunchKernelGGL(k_fast_kp<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const float2*)ring.data,
+                             ring_stride, nslot, (float2*)pl.data);
+        else {
+          int e2 = by_ncomp([&](auto nc) {
+            hipLaunchKernelGGL(k_fast_kp16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, P.st, V, ring,
+                               nslot, pl);
+          });
+          if (e2) return e2;
+        }
+        LYM_HIP(hipGetLastError());
+        return (int)LYM_OK;
+      });
+      if (e) return e;
       EpiArgs a1;
       a1.out = t;
       a1.st = P.st;
       a1.rhs_stride = n;
-      int e = wilson_launch(EPI_CGM_K1, d, prec, true, pl, a1, redctl(P), st, nrhs);
-      if (!e) e = launch_fin(FK_CGM_A, P.st, P.partials, gw, 2, nullptr, st, nrhs, P.ctl);
+      e = kt_launch(st, ("wilson_cgm_k1_" + sfx).c_str(), wilson_bytes_site(d, prec, 2) * sites, -1,
+                    [&]() { return wilson_launch(EPI_CGM_K1, d, prec, true, pl, a1, redctl(P), st, nrhs); });
+      if (!e)
+        e = kt_launch(st, "finalize", 0.0, -1,
+                      [&]() { return launch_fin(FK_CGM_A, P.st, P.partials, gw, 2, nullptr, st, nrhs, P.ctl); });
       if (e) return e;
       EpiArgs a2;
       a2.aux0 = pl;
@@ -1294,20 +1410,27 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
       a2.nslot = nslot;
       a2.rhs_stride = n;
       a2.ring_rhs_stride = ring_stride;
-      e = wilson_launch(EPI_CGM_K2F, d, prec, true, t, a2, redctl(P), st, nrhs);
-      if (!e) e = launch_fin(FK_CGM_BF, P.st, P.partials, gw, 1, nullptr, st, nrhs, P.ctl);
+      e = kt_launch(st, ("wilson_cgm_k2f_" + sfx).c_str(), wilson_bytes_site(d, prec, 4) * sites, -1,
+                    [&]() { return wilson_launch(EPI_CGM_K2F, d, prec, true, t, a2, redctl(P), st, nrhs); });
+      if (!e)
+        e = kt_launch(st, "finalize", 0.0, -1,
+                      [&]() { return launch_fin(FK_CGM_BF, P.st, P.partials, gw, 1, nullptr, st, nrhs, P.ctl); });
       if (e) return e;
     }
-    if (prec == P64)
-      hipLaunchKernelGGL(k_fast_k3, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, d.ncomp, V, ring, ring_stride, nslot,
-                         P.xs_dev, P.ps_dev, nshift);
-    else if (prec == P32)
-      hipLaunchKernelGGL(k_fast_k3s<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, d.ncomp, V, ring, ring_stride,
-                         nslot, P.xs_dev, P.ps_dev, nshift);
-    else
-      hipLaunchKernelGGL(k_fast_k3s<P16>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, d.ncomp, V, ring, ring_stride,
-                         nslot, P.xs_dev, P.ps_dev, nshift);
-    LYM_HIP(hipGetLastError());
+    int e = kt_launch(st, ("shift_update_" + sfx).c_str(), -1.0, ci, [&]() {
+      if (prec == P64)
+        hipLaunchKernelGGL(k_fast_k3<P64>, dim3(gf, nrhs), dim3(kBlock), k3_lds_bytes(P64, nshift), st, P.st, d.ncomp, V, ring, ring_stride,
+                           nslot, P.xs_dev, P.ps_dev, nshift);
+      else if (prec == P32)
+        hipLaunchKernelGGL(k_fast_k3<P32>, dim3(gf, nrhs), dim3(kBlock), k3_lds_bytes(P32, nshift), st, P.st, d.ncomp, V, ring, ring_stride,
+                           nslot, P.xs_dev, P.ps_dev, nshift);
+      else
+        hipLaunchKernelGGL(k_fast_k3<P16>, dim3(gf, nrhs), dim3(kBlock), k3_lds_bytes(P16, nshift), st, P.st, d.ncomp, V, ring, ring_stride,
+                           nslot, P.xs_dev, P.ps_dev, nshift);
+      LYM_HIP(hipGetLastError());
+      return (int)LYM_OK;
+    });
+    if (e) return e;
     return launch_fin(FK_FAST_K3ADV, P.st, nullptr, 0, 0, nullptr, st, nrhs, P.ctl);
   };
   // poll the batch control block: (ndone, nswitch, nrhs, -).  Each in-flight
@@ -1365,6 +1488,27 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   out_states.resize(nrhs);
   LYM_HIP(hipMemcpyAsync(out_states.data(), P.st, sizeof(SolverState) * nrhs, hipMemcpyDeviceToHost, st));
   LYM_HIP(hipStreamSynchronize(st));
+  if (g_kt.on) {
+    // first iteration of every chunk, then the shift-update bytes of chunk c:
+    // per RHS, the block's residuals (nb of them) and 4 FP64 vectors of
+    // every shift still active at its start
+    std::vector<int> k0(chunk_log.size() + 1, 0);
+    for (size_t c = 0; c < chunk_log.size(); ++c) k0[c + 1] = k0[c] + chunk_log[c].second;
+    rc = kt_collect([&](int c) {
+      double b = 0.0;
+      for (int r = 0; r < nrhs; ++r) {
+        const SolverState& h = out_states[r];
+        const int nb = std::min(chunk_log[c].second, h.iter - k0[c]);
+        if (nb <= 0) continue;
+        int act = 0;
+        for (int k = 0; k < nshift; ++k)
+          if (h.conv_iter[k] < 0 || h.conv_iter[k] > k0[c]) ++act;
+        b += (double)V * (nb * vec_bytes_site(d, chunk_log[c].first) + act * 4.0 * vec_bytes_site(d, P64));
+      }
+      return b;
+    });
+    if (rc) return rc;
+  }
   return LYM_OK;
 }
 
