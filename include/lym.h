@@ -119,6 +119,17 @@ int lym_wilson_apply_square(const lym_wilson* op, int prec, double shift, const 
  * (lattice.py:375-376). `scratch` >= lym_reduce_scratch_bytes(n) bytes. */
 size_t lym_reduce_scratch_bytes(int64_t n);
 int lym_dot(int prec, int64_t n, const void* x, const void* y, double* result, void* scratch, void* stream);
+/* lym_dot without a scratch argument and without allocation: no counterpart in
+ * the reference; for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).
+ * result_dev[0] + i result_dev[1] = sum conj(x) y in FP64 with lym_dot's
+ * kernels and summation order (bitwise the same value), written to DEVICE
+ * memory, asynchronously on `stream` (no synchronisation, no host copy).
+ * Scratch: a static 32 KiB device array per GPU (2048 block partials; any n,
+ * the reduction grid is capped there), held until the process ends; its
+ * device address is looked up by lym_init or the first call on a device.
+ * Calls that overlap on one GPU share that array and race. */
+int lym_dot_noalloc(int prec, int64_t n, const void* x, const void* y, double* result_dev, void* stream);
 /* y = a x + b y (complex scalars); replaces the solver updates written as
  * tensor arithmetic in dirac_solver.py:33-45 and dirac_solver.py:83-97. */
 int lym_axpby(int prec, int64_t n, double a_re, double a_im, const void* x, double b_re, double b_im, void* y,

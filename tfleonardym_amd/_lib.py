@@ -117,6 +117,7 @@ SIGNATURES = {
     "lym_wilson_apply_square": (_i32, [ctypes.POINTER(LymWilson), _i32, _f64, _vp, _vp, _vp, _vp]),
     "lym_reduce_scratch_bytes": (_sz, [_i64]),
     "lym_dot": (_i32, [_i32, _i64, _vp, _vp, ctypes.POINTER(_f64), _vp, _vp]),
+    "lym_dot_noalloc": (_i32, [_i32, _i64, _vp, _vp, _vp, _vp]),
     "lym_axpby": (_i32, [_i32, _i64, _f64, _f64, _vp, _f64, _f64, _vp, _vp]),
     "lym_gamma5": (_i32, [_i32, _i32, _i64, _vp, _vp, _vp]),
     "lym_spin_axpby": (_i32, [_i32, _i64, ctypes.POINTER(_f64), _vp, _vp, _vp, _vp]),

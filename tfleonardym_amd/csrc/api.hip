@@ -251,7 +251,8 @@ int lym_init(int device) {
   int cur = -1;
   LYM_HIP(hipGetDevice(&cur));
   if (cur != device) LYM_HIP(hipSetDevice(device));
-  return LYM_OK;
+  void* sp = nullptr;  // device address of lym_dot_noalloc's static scratch on this device
+  return noalloc_scratch(&sp);
 }
 
 const char* lym_last_error(void) { return g_err.c_str(); }
@@ -325,6 +326,14 @@ int lym_dot(int prec, int64_t n, const void* x, const void* y, double* result, v
   LYM_HIP(hipMemcpyAsync(result, res, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   LYM_HIP(hipStreamSynchronize(st));
   return LYM_OK;
+}
+
+int lym_dot_noalloc(int prec, int64_t n, const void* x, const void* y, double* result_dev, void* stream) {
+  if (!x || !y || !result_dev || n < 0) {
+    set_error("lym_dot_noalloc: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  return blas_dot_noalloc(prec, n, x, y, result_dev, (hipStream_t)stream);
 }
 
 int lym_axpby(int prec, int64_t n, double a_re, double a_im, const void* x, double b_re, double b_im, void* y,

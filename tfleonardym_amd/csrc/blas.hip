@@ -3,6 +3,8 @@
 // depend only on n and the data.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "lym_internal.h"
 
 namespace lym {
@@ -130,6 +132,69 @@ int blas_dot(int prec, int64_t n, const void* x, const void* y, double* out_dev,
   }
   LYM_HIP(hipGetLastError());
   return launch_finalize(FK_SUM, nullptr, rc.partials, g, 2, out_dev, st);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free dot product (lym_dot_noalloc).  Its scratch, the block
+// partials of k_dot (2 per block, at most kNoAllocBlocks blocks: red_grid
+// caps the grid there for any n) and nothing else, lives in this file-scope
+// device array: 32 KiB that every GPU the library runs on holds until the
+// process ends.  In host code `scratch` names a host-side stand-in, not the
+// device memory: its device address is looked up once per device with
+// hipGetSymbolAddress (lym_init, or the first call on a device) and only that
+// address is handed to the kernels.  Every call on a GPU shares that GPU's
+// array, so calls that overlap on one GPU race; the kernels overwrite every
+// partial they read, so nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocBlocks = 2048;
+static __device__ __align__(256) double scratch[2 * kNoAllocBlocks];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices];  // device address of `scratch`, per device
+
+// device address of `scratch` on the current device (looked up once)
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  LYM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("lym_dot_noalloc: device index above 63");
+    return LYM_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return LYM_OK;
+}
+
+int blas_dot_noalloc(int prec, int64_t n, const void* x, const void* y, double* out_dev, hipStream_t st) {
+  void* sp = nullptr;
+  int rc = noalloc_scratch(&sp);
+  if (rc) return rc;
+  const int g = red_grid(n);
+  if (g > kNoAllocBlocks) {  // red_grid caps the grid: cannot happen
+    set_error("lym_dot_noalloc: reduction grid above the static scratch");
+    return LYM_ERR_ARG;
+  }
+  double* partials = (double*)sp;
+  const RedCtl red{partials, nullptr};
+  if (prec == P64)
+    hipLaunchKernelGGL(k_dot<double2>, dim3(g), dim3(kBlock), 0, st, n, (const double2*)x, (const double2*)y, red,
+                       out_dev);
+  else if (prec == P32)
+    hipLaunchKernelGGL(k_dot<float2>, dim3(g), dim3(kBlock), 0, st, n, (const float2*)x, (const float2*)y, red,
+                       out_dev);
+  else {
+    set_error("lym_dot_noalloc: precision must be F64 or F32");
+    return LYM_ERR_ARG;
+  }
+  LYM_HIP(hipGetLastError());
+  return launch_finalize(FK_SUM, nullptr, partials, g, 2, out_dev, st);
 }
 
 int blas_diff_norm2(int prec, int64_t n, const void* a, const void* b, double* out_dev, RedCtl rc, hipStream_t st) {
