@@ -253,6 +253,23 @@ int lym_mr_multishift(const lym_wilson* op, int nshift, const double* shifts, in
                       void* const* xs, int max_steps, double precision, double omega, void* workspace,
                       size_t workspace_bytes, lym_solve_stats* stats, void* stream);
 
+/* BiCGStab exactly as biconjugate_gradient.solve (dirac_solver.py:56-110) on
+ * A = Op + shift (square = 0) or A = Op^2 + shift (square = 1), Op = D, or
+ * Q when op->hermitian: x0 = initial guess (NULL: b), r = b - A x0, shadow
+ * vector r_hat = x0 + A x0 (dirac_solver.py:67-70), alpha = omega = rho = 1,
+ * p = nu = 0, the standard updates, stop when |<r,r>| < precision
+ * (absolute, iterated, dirac_solver.py:96-103) or after max_steps.  The
+ * reference's r = x0 - A x0 (correct only for x0 = b) is not reproduced.
+ * Replaces the reference's solver for Propagator.apply (dirac.py:670-672) and
+ * for the leading negative pole of the multishift solver
+ * (dirac_solver.py:26-30).  F64; device-resident (no host round trip per
+ * iteration); x must not alias b.  stats->iterations = iterations done.
+ * Synchronous. */
+size_t lym_bicgstab_workspace_bytes(const lym_wilson* op);
+int lym_bicgstab(const lym_wilson* op, int square, double shift, const void* b, const void* x0, void* x,
+                 int max_steps, double precision, void* workspace, size_t workspace_bytes, lym_solve_stats* stats,
+                 void* stream);
+
 /* Multishift conjugate gradient (Jegerlehner, hep-lat/9612014) on
  * A = Q^2 + s_i (requires op->hermitian): x_i = (Q^2 + s_i)^-1 b, all shifts
  * >= 0, any order (the base is the smallest).  Stops each shift when its

@@ -22,7 +22,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def setup(geometry, nc, kappa, kind="near_identity", eps=0.3, bc=-1.0, rep="fundamental", seed=77):
+def setup(geometry, nc, kappa, kind="near_identity", eps=0.3, bc=-1.0, rep="fundamental", seed=77, herm=True):
     from tfleonardym_amd import boundary_conditions, dirac, lattice
     U = fields.gauge_field(geometry, nc, seed=seed, kind=kind, eps=eps)
     cfg = lattice.Configuration(U, geometry, nc, len(geometry))
@@ -30,8 +30,8 @@ def setup(geometry, nc, kappa, kind="near_identity", eps=0.3, bc=-1.0, rep="fund
     if bc is not None:
         boundary_conditions.set_antiperiodic_field_in_t_direction(geometry, factor=bc)
         prov = boundary_conditions.apply_antiboundary_conditions_in_t_direction
-    op = dirac.DiracWilsonOperator(cfg, kappa, True, representation=rep, boundary_condition_provider=prov)
-    oop = ow.WilsonOracle(U, geometry, kappa, True, rep, bc_factor=bc)
+    op = dirac.DiracWilsonOperator(cfg, kappa, herm, representation=rep, boundary_condition_provider=prov)
+    oop = ow.WilsonOracle(U, geometry, kappa, herm, rep, bc_factor=bc)
     return op, oop
 
 
@@ -451,6 +451,46 @@ def test_mr_negative_shift_and_bicgstab():
     assert abs(bi.last_steps - step) <= 1
     assert rel(x, xr) < 1e-7
     assert rel(x, xs[-1].cpu().numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("geometry,nc,herm,square,shift,guess", [
+    ((4, 4, 4, 4), 2, True, 1, 0.05, False),   # SquareOperator(Q) + shift (the negative-pole path's form)
+    ((4, 4, 4, 8), 3, False, 0, 0.0, False),   # D itself, non-Hermitian
+    ((4, 4, 4, 8), 3, False, 0, 0.0, True),    # D with an initial guess
+    ((4, 4, 4, 8), 3, True, 0, 0.0, True),     # Q, Hermitian indefinite, with an initial guess
+    ((8, 8), 2, True, 1, 0.0, False),          # 2D, D^dag D (Propagator's form)
+    ((6, 4), 3, False, 0, 0.3, True),          # 2D, D + shift, initial guess
+])
+def test_bicgstab_device(geometry, nc, herm, square, shift, guess):
+    """lym_bicgstab (device-resident BiCGStab, dirac_solver.py:56-110) against
+    the oracle's restatement of the same recurrence on the same inputs: step
+    index within 1, solution within 1e-7, the absolute stop rule met.  On the
+    indefinite Q (no square) BiCGStab's convergence history is erratic and
+    rounding moves the stopping step (153 against the oracle's 128 in one
+    measured case), so there only the solution is compared."""
+    from tfleonardym_amd import dirac, dirac_solver
+    print(f"case bicgstab geometry={geometry} nc={nc} herm={herm} square={square} shift={shift} guess={guess}",
+          flush=True)
+    op, oop = setup(geometry, nc, 0.12, kind="near_identity", eps=0.4, herm=herm)
+    ns = 4 if len(geometry) == 4 else 2
+    b = fields.gaussian_spinor(geometry, ns, nc, seed=9)
+    x0 = fields.gaussian_spinor(geometry, ns, nc, seed=10) * 0.1 if guess else None
+    A = dirac.SquareOperator(op) if square else op
+    if shift:
+        A = dirac.ShiftedOperator(A, shift)
+    oapply = (lambda v: oop.apply_square(v, shift)) if square else (lambda v: oop.apply(v) + shift * v)
+    prec = 1e-16 * float(np.vdot(b, b).real)
+    bi = dirac_solver.biconjugate_gradient(2000, prec, verbose=False)
+    x = bi.solve(A, b, x0)
+    assert bi.last_stats is not None  # the device path ran
+    xr, step, ok = osv.bicgstab(oapply, b, 2000, prec, initial_guess=x0)
+    assert ok and bi.last_status == "converged", (bi.last_stats, step)
+    if not (herm and not square):
+        assert abs(bi.last_steps - step) <= 1, (bi.last_steps, step)
+    xn = x.cpu().numpy()
+    assert rel(xn, xr) < 1e-7
+    r = b - oapply(xn)
+    assert abs(np.vdot(r, r)) < 10 * prec  # the iterated residual met the absolute rule; the true one follows
 
 
 def test_rational_approximation_apply():

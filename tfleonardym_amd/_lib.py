@@ -137,6 +137,9 @@ SIGNATURES = {
                                  ctypes.POINTER(LymSolveStats), _vp]),
     "lym_cgm": (_i32, [ctypes.POINTER(LymWilson), _i32, _i32, ctypes.POINTER(_f64), _vp, ctypes.POINTER(_vp),
                        _f64, _i32, _vp, _sz, ctypes.POINTER(LymSolveStats), _vp]),
+    "lym_bicgstab_workspace_bytes": (_sz, [ctypes.POINTER(LymWilson)]),
+    "lym_bicgstab": (_i32, [ctypes.POINTER(LymWilson), _i32, _f64, _vp, _vp, _vp, _i32, _f64, _vp, _sz,
+                            ctypes.POINTER(LymSolveStats), _vp]),
     "lym_kernel_timing": (_i32, [_i32]),
     "lym_kernel_timing_read": (_i32, [ctypes.POINTER(LymKernelTime), _i32, ctypes.POINTER(_i32)]),
     "lym_cgm_batched_workspace_bytes": (_sz, [ctypes.POINTER(LymWilson), _i32, _i32, _i32]),

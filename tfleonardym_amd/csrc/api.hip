@@ -230,6 +230,9 @@ int solve_cgm(const OpDesc& d, int prec, int nshift, const double* shifts, const
 int solve_cgm_batched(const OpDesc& d, int prec, int nrhs, int nshift, const double* shifts, const void* const* bs,
                       void* const* xs, double tol, int max_iter, void* ws, size_t ws_bytes, lym_solve_stats* stats,
                       hipStream_t st);
+size_t bicgstab_workspace_bytes(const OpDesc& d);
+int solve_bicgstab(const OpDesc& d, int square, double shift, const void* b, const void* x0, void* x, int max_steps,
+                   double precision, void* ws, size_t ws_bytes, lym_solve_stats* stats, hipStream_t st);
 int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const void* b, void* const* xs,
              int max_steps, double precision, double omega, void* ws, size_t ws_bytes, lym_solve_stats* stats,
              hipStream_t st);
@@ -629,6 +632,27 @@ int lym_cgm_batched(const lym_wilson* op, int prec, int nrhs, int nshift, const 
   if (stats) std::memset(stats, 0, sizeof(*stats) * nrhs);
   return solve_cgm_batched(d, prec, nrhs, nshift, shifts, bs, xs, tol, max_iter, workspace, workspace_bytes, stats,
                            (hipStream_t)stream);
+}
+
+size_t lym_bicgstab_workspace_bytes(const lym_wilson* op) {
+  OpDesc d;
+  if (make_desc(op, &d)) return 0;
+  return bicgstab_workspace_bytes(d);
+}
+
+int lym_bicgstab(const lym_wilson* op, int square, double shift, const void* b, const void* x0, void* x,
+                 int max_steps, double precision, void* workspace, size_t workspace_bytes, lym_solve_stats* stats,
+                 void* stream) {
+  OpDesc d;
+  int rc = make_desc(op, &d);
+  if (rc) return rc;
+  if (!b || !x || max_steps < 1 || (square != 0 && square != 1) || !std::isfinite(shift) || x == b) {
+    set_error("lym_bicgstab: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  return solve_bicgstab(d, square, shift, b, x0, x, max_steps, precision, workspace, workspace_bytes, stats,
+                        (hipStream_t)stream);
 }
 
 int lym_kernel_timing(int enable) { return kernel_timing(enable); }

@@ -248,4 +248,71 @@ __device__ inline void mr_fin_rr(SolverState* st, double rr) {
   }
 }
 
+// ---- BiCGStab (dirac_solver.py:56-110), complex scalars (re, im) -----------
+__device__ inline void cdiv(const double* a, const double* b, double* out) {
+  const double d = b[0] * b[0] + b[1] * b[1];
+  const double re = (a[0] * b[0] + a[1] * b[1]) / d, im = (a[1] * b[0] - a[0] * b[1]) / d;
+  out[0] = re;
+  out[1] = im;
+}
+__device__ inline void cmulv(const double* a, const double* b, double* out) {
+  const double re = a[0] * b[0] - a[1] * b[1], im = a[0] * b[1] + a[1] * b[0];
+  out[0] = re;
+  out[1] = im;
+}
+// beta = (rho_next / rho) (alpha / omega)   (dirac_solver.py:80)
+__device__ inline bool bicg_beta(SolverState* st) {
+  if (st->bi_rho[0] == 0.0 && st->bi_rho[1] == 0.0) return false;
+  if (st->bi_omega[0] == 0.0 && st->bi_omega[1] == 0.0) return false;
+  double q1[2], q2[2];
+  cdiv(st->bi_rho_next, st->bi_rho, q1);
+  cdiv(st->bi_alpha, st->bi_omega, q2);
+  cmulv(q1, q2, st->bi_beta);
+  return isfinite(st->bi_beta[0]) && isfinite(st->bi_beta[1]);
+}
+__device__ inline void bicg_stop(SolverState* st, int status) {
+  st->status = status;
+  st->phase = PH_DONE;
+}
+// after the initial residual: alpha = omega = rho = 1 (dirac_solver.py:71-76),
+// rho_next = <r_hat, r> of the first iteration.  A zero residual (b = A x0
+// exactly) is solved at once.
+__device__ inline void bicg_fin_init(SolverState* st, double rr, double re, double im) {
+  st->bi_rho[0] = st->bi_alpha[0] = st->bi_omega[0] = 1.0;
+  st->bi_rho[1] = st->bi_alpha[1] = st->bi_omega[1] = 0.0;
+  st->bi_rho_next[0] = re;
+  st->bi_rho_next[1] = im;
+  st->rr = rr;
+  st->bb = rr;
+  if (!isfinite(rr)) return bicg_stop(st, ST_BREAKDOWN);
+  if (rr == 0.0) return bicg_stop(st, ST_CONVERGED);
+  if (!bicg_beta(st)) return bicg_stop(st, ST_BREAKDOWN);
+}
+// alpha = rho_next / <r_hat, nu>   (dirac_solver.py:88)
+__device__ inline void bicg_fin_a(SolverState* st, double re, double im) {
+  const double g[2] = {re, im};
+  if ((re == 0.0 && im == 0.0) || !isfinite(re) || !isfinite(im)) return bicg_stop(st, ST_BREAKDOWN);
+  cdiv(st->bi_rho_next, g, st->bi_alpha);
+}
+// omega = <t, s> / <t, t>   (dirac_solver.py:91); t = 0 (then s = 0) gives 0
+__device__ inline void bicg_fin_w(SolverState* st, double re, double im, double tt) {
+  if (!isfinite(tt) || !isfinite(re) || !isfinite(im)) return bicg_stop(st, ST_BREAKDOWN);
+  st->bi_omega[0] = tt > 0.0 ? re / tt : 0.0;
+  st->bi_omega[1] = tt > 0.0 ? im / tt : 0.0;
+}
+// after r = s - omega t: stop when |<r, r>| < precision (absolute,
+// dirac_solver.py:96-103); else rho = rho_next, rho_next = <r_hat, r>, beta
+__device__ inline void bicg_fin_r(SolverState* st, double rr, double re, double im) {
+  st->iter += 1;
+  st->rr = rr;
+  if (!isfinite(rr)) return bicg_stop(st, ST_BREAKDOWN);
+  if (fabs(rr) < st->tol2) return bicg_stop(st, ST_CONVERGED);
+  if (st->iter >= st->max_iter) return bicg_stop(st, ST_MAXITER);
+  st->bi_rho[0] = st->bi_rho_next[0];
+  st->bi_rho[1] = st->bi_rho_next[1];
+  st->bi_rho_next[0] = re;
+  st->bi_rho_next[1] = im;
+  if (!bicg_beta(st)) return bicg_stop(st, ST_BREAKDOWN);
+}
+
 }  // namespace lym

@@ -95,6 +95,8 @@ enum EpiKind : int {
   EPI_RU1 = 8,       // reliable update (complex128, gate RU): out = Op in
   EPI_RU2 = 9,       // reliable update: r = bs - (Op in + sigma0 aux0) -> out (complex64); reduce |r|^2
   EPI_RU2H = 10,     // the same with out in half storage (per-site norms)
+  EPI_GSTORE = 11,   // out = Op in, only while st->phase == PH_K1 (BiCGStab)
+  EPI_GAXPY = 12,    // out = Op in + sigma aux0, only while st->phase == PH_K1
 };
 int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, const EpiArgs& a, RedCtl rc,
                   hipStream_t stream, int nrhs = 1);
@@ -132,6 +134,10 @@ enum FinKind : int {
   FK_CG_A = 9,        // gate K1|KP: mixed CG step length (counts a breakdown as done)
   FK_CG_B = 10,       // gate K2: mixed CG beta / reliable-update decision
   FK_CG_RU = 11,      // gate RU: mixed CG after the FP64 true residual
+  FK_BICG_INIT = 12,  // gate K1: BiCGStab rho_next = <r_hat, r>, beta
+  FK_BICG_A = 13,     // gate K1: alpha = rho_next / <r_hat, nu>
+  FK_BICG_W = 14,     // gate K1: omega = <t, s> / <t, t>
+  FK_BICG_R = 15,     // gate K1: stop test on |<r, r>|, rho, beta
 };
 // nb finalizer blocks: block i handles state st + i, partials + i * nr * G, out + i * nr
 int launch_finalize(FinKind kind, SolverState* st, const double* partials, int G, int nr, double* out,

@@ -97,6 +97,10 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       case FK_CG_A: run = (ph == PH_KP); break;
       case FK_CG_B: run = (ph == PH_K2); break;
       case FK_CG_RU: run = (ph == PH_RU); break;
+      case FK_BICG_INIT:
+      case FK_BICG_A:
+      case FK_BICG_W:
+      case FK_BICG_R: run = (ph == PH_K1); break;
       default: run = true;
     }
   }
@@ -185,6 +189,10 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       break;
     case FK_MR: mr_fin(st, sums[0], sums[1], sums[2]); break;
     case FK_MR_RR: mr_fin_rr(st, sums[0]); break;
+    case FK_BICG_INIT: bicg_fin_init(st, sums[0], sums[1], sums[2]); break;
+    case FK_BICG_A: bicg_fin_a(st, sums[0], sums[1]); break;
+    case FK_BICG_W: bicg_fin_w(st, sums[0], sums[1], sums[2]); break;
+    case FK_BICG_R: bicg_fin_r(st, sums[0], sums[1], sums[2]); break;
     case FK_CG_A:
     case FK_CG_B:
     case FK_CG_RU:
@@ -685,6 +693,106 @@ __global__ __launch_bounds__(kBlock) void k_mr_k3(const SolverState* st, int64_t
     red[0] += rv.x * rv.x + rv.y * rv.y;
   }
   block_partials<1>(red, partials);
+}
+
+// ---- BiCGStab (complex128, dirac_solver.py:56-110) --------------------------
+// Every kernel runs only while the solve is running (phase PH_K1).
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// r = b - A x0 (init holds A x0), r_hat = x0 + A x0, p = nu = 0;
+// partials |r|^2, <r_hat, r> (re, im)
+__global__ __launch_bounds__(kBlock) void k_bicg_init(const SolverState* st, int64_t n, const double2* __restrict__ b,
+                                                      const double2* __restrict__ x, const double2* __restrict__ init,
+                                                      double2* __restrict__ r, double2* __restrict__ rh,
+                                                      double2* __restrict__ p, double2* __restrict__ nu,
+                                                      double* partials) {
+  double v[3] = {0.0, 0.0, 0.0};
+  if (st->phase == PH_K1) {
+    LYM_GRID_STRIDE(i, n) {
+      const double2 a = init[i], bv = b[i], xv = x[i];
+      const double2 rv = make_double2(bv.x - a.x, bv.y - a.y);
+      const double2 hv = make_double2(xv.x + a.x, xv.y + a.y);
+      r[i] = rv;
+      rh[i] = hv;
+      p[i] = make_double2(0.0, 0.0);
+      nu[i] = make_double2(0.0, 0.0);
+      v[0] += rv.x * rv.x + rv.y * rv.y;
+      v[1] += hv.x * rv.x + hv.y * rv.y;  // <r_hat, r> = sum conj(r_hat) r
+      v[2] += hv.x * rv.y - hv.y * rv.x;
+    }
+  }
+  block_partials<3>(v, partials);
+}
+
+// p = r + beta (p - omega nu)
+__global__ __launch_bounds__(kBlock) void k_bicg_p(const SolverState* st, int64_t n, const double2* __restrict__ r,
+                                                   const double2* __restrict__ nu, double2* __restrict__ p) {
+  if (st->phase != PH_K1) return;
+  const double2 be = make_double2(st->bi_beta[0], st->bi_beta[1]);
+  const double2 om = make_double2(st->bi_omega[0], st->bi_omega[1]);
+  LYM_GRID_STRIDE(i, n) {
+    const double2 on = cmul2(om, nu[i]);
+    const double2 pv = p[i];
+    const double2 q = cmul2(be, make_double2(pv.x - on.x, pv.y - on.y));
+    const double2 rv = r[i];
+    p[i] = make_double2(rv.x + q.x, rv.y + q.y);
+  }
+}
+
+// partials <a, c> (re, im) = sum conj(a) c, and |a|^2
+__global__ __launch_bounds__(kBlock) void k_bicg_dots(const SolverState* st, int64_t n, const double2* __restrict__ a,
+                                                      const double2* __restrict__ c, double* partials) {
+  double v[3] = {0.0, 0.0, 0.0};
+  if (st->phase == PH_K1) {
+    LYM_GRID_STRIDE(i, n) {
+      const double2 av = a[i], cv = c[i];
+      v[0] += av.x * cv.x + av.y * cv.y;
+      v[1] += av.x * cv.y - av.y * cv.x;
+      v[2] += av.x * av.x + av.y * av.y;
+    }
+  }
+  block_partials<3>(v, partials);
+}
+
+// s = r - alpha nu
+__global__ __launch_bounds__(kBlock) void k_bicg_s(const SolverState* st, int64_t n, const double2* __restrict__ r,
+                                                   const double2* __restrict__ nu, double2* __restrict__ s) {
+  if (st->phase != PH_K1) return;
+  const double2 al = make_double2(st->bi_alpha[0], st->bi_alpha[1]);
+  LYM_GRID_STRIDE(i, n) {
+    const double2 an = cmul2(al, nu[i]);
+    const double2 rv = r[i];
+    s[i] = make_double2(rv.x - an.x, rv.y - an.y);
+  }
+}
+
+// x += alpha p + omega s ; r = s - omega t ; partials |r|^2, <r_hat, r>
+__global__ __launch_bounds__(kBlock) void k_bicg_xr(const SolverState* st, int64_t n, double2* __restrict__ x,
+                                                    const double2* __restrict__ p, const double2* __restrict__ s,
+                                                    const double2* __restrict__ t, const double2* __restrict__ rh,
+                                                    double2* __restrict__ r, double* partials) {
+  double v[3] = {0.0, 0.0, 0.0};
+  if (st->phase == PH_K1) {
+    const double2 al = make_double2(st->bi_alpha[0], st->bi_alpha[1]);
+    const double2 om = make_double2(st->bi_omega[0], st->bi_omega[1]);
+    LYM_GRID_STRIDE(i, n) {
+      const double2 sv = s[i];
+      const double2 ap = cmul2(al, p[i]), os = cmul2(om, sv), ot = cmul2(om, t[i]);
+      double2 xv = x[i];
+      xv.x += ap.x + os.x;
+      xv.y += ap.y + os.y;
+      x[i] = xv;
+      const double2 rv = make_double2(sv.x - ot.x, sv.y - ot.y);
+      r[i] = rv;
+      const double2 hv = rh[i];
+      v[0] += rv.x * rv.x + rv.y * rv.y;
+      v[1] += hv.x * rv.x + hv.y * rv.y;
+      v[2] += hv.x * rv.y - hv.y * rv.x;
+    }
+  }
+  block_partials<3>(v, partials);
 }
 
 // ---- FP64 helpers for true residuals and defect correction -------------------
@@ -1959,6 +2067,122 @@ int solve_cgm(const OpDesc& d, int prec, int nshift, const double* shifts, const
               int max_iter, void* ws, size_t ws_bytes, lym_solve_stats* stats, hipStream_t st) {
   const void* bs[1] = {b};
   return solve_cgm_batched(d, prec, 1, nshift, shifts, bs, xs, tol, max_iter, ws, ws_bytes, stats, st);
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab (dirac_solver.py:56-110) on A = Op + shift or A = Op^2 + shift
+// (square), Op = D or Q, complex128, device-resident: every scalar lives in
+// the SolverState, an iteration is a fixed launch sequence (chunks of 8
+// iterations, HIP graphs), and the host polls four ints per chunk.
+// Workspace: state, partials, and r, r_hat, p, nu, s, t, tmp.
+// ---------------------------------------------------------------------------
+struct BicgParts {
+  SolverState* st;
+  double* partials;
+  Field r, rh, p, nu, s, t, tmp;
+};
+static size_t bicg_carve(const OpDesc& d, char* base, BicgParts* bp) {
+  Carver c{base};
+  BicgParts tmp;
+  BicgParts& B = bp ? *bp : tmp;
+  B.st = (SolverState*)c.take(sizeof(SolverState));
+  B.partials = (double*)c.take(sizeof(double) * kMaxRed * (size_t)max_red_blocks(d.lat.V));
+  for (Field* f : {&B.r, &B.rh, &B.p, &B.nu, &B.s, &B.t, &B.tmp}) *f = take_field(c, d, P64);
+  return c.off + 256;
+}
+
+size_t bicgstab_workspace_bytes(const OpDesc& d) { return bicg_carve(d, nullptr, nullptr); }
+
+int solve_bicgstab(const OpDesc& d, int square, double shift, const void* b, const void* x0, void* x, int max_steps,
+                   double precision, void* ws, size_t ws_bytes, lym_solve_stats* stats, hipStream_t st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  graph_new_solve();
+  const size_t need = bicgstab_workspace_bytes(d);
+  if (!ws || ws_bytes < need) {
+    set_error("BiCGStab workspace too small: need " + std::to_string(need) + " bytes");
+    return LYM_ERR_ARG;
+  }
+  BicgParts B;
+  char* base = (char*)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+  bicg_carve(d, base, &B);
+  const int64_t n = field_elems(d);
+  SolverState h;
+  std::memset(&h, 0, sizeof(h));
+  h.phase = PH_K1;
+  h.max_iter = max_steps;
+  h.tol2 = precision;
+  h.bi_shift = shift;
+  LYM_HIP(hipMemcpyAsync(B.st, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  LYM_HIP(hipMemcpyAsync(x, x0 ? x0 : b, (size_t)n * 16, hipMemcpyDeviceToDevice, st));
+  const RedCtl red{B.partials, nullptr};
+  const RedCtl nored{nullptr, nullptr};
+  // out = A in (tmp: Op in when squared); gated on a running solve
+  auto applyA = [&](Field in, Field out, hipStream_t s) -> int {
+    Field src = in;
+    if (square) {
+      EpiArgs a;
+      a.out = B.tmp;
+      a.st = B.st;
+      const int e = wilson_launch(EPI_GSTORE, d, P64, true, in, a, nored, s);
+      if (e) return e;
+      src = B.tmp;
+    }
+    EpiArgs a2;
+    a2.out = out;
+    a2.aux0 = in;
+    a2.sigma = shift;
+    a2.st = B.st;
+    return wilson_launch(EPI_GAXPY, d, P64, true, src, a2, nored, s);
+  };
+  const int gr = red_grid_flat(n);
+  const int gf = flat_grid(n);
+  Field xf;
+  xf.data = x;
+  int rc = applyA(xf, B.nu, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_bicg_init, dim3(gr), dim3(kBlock), 0, st, B.st, n, (const double2*)b, (const double2*)x,
+                     (const double2*)B.nu.data, (double2*)B.r.data, (double2*)B.rh.data, (double2*)B.p.data,
+                     (double2*)B.nu.data, B.partials);
+  LYM_HIP(hipGetLastError());
+  if ((rc = launch_finalize(FK_BICG_INIT, B.st, B.partials, gr, 3, nullptr, st))) return rc;
+  const int per_chunk = 8;
+  auto chunk = [&](hipStream_t s) -> int {
+    for (int j = 0; j < per_chunk; ++j) {
+      hipLaunchKernelGGL(k_bicg_p, dim3(gf), dim3(kBlock), 0, s, B.st, n, (const double2*)B.r.data,
+                         (const double2*)B.nu.data, (double2*)B.p.data);
+      LYM_HIP(hipGetLastError());
+      int e = applyA(B.p, B.nu, s);
+      if (e) return e;
+      hipLaunchKernelGGL(k_bicg_dots, dim3(gr), dim3(kBlock), 0, s, B.st, n, (const double2*)B.rh.data,
+                         (const double2*)B.nu.data, B.partials);
+      LYM_HIP(hipGetLastError());
+      if ((e = launch_finalize(FK_BICG_A, B.st, B.partials, gr, 3, nullptr, s))) return e;
+      hipLaunchKernelGGL(k_bicg_s, dim3(gf), dim3(kBlock), 0, s, B.st, n, (const double2*)B.r.data,
+                         (const double2*)B.nu.data, (double2*)B.s.data);
+      LYM_HIP(hipGetLastError());
+      if ((e = applyA(B.s, B.t, s))) return e;
+      hipLaunchKernelGGL(k_bicg_dots, dim3(gr), dim3(kBlock), 0, s, B.st, n, (const double2*)B.t.data,
+                         (const double2*)B.s.data, B.partials);
+      LYM_HIP(hipGetLastError());
+      if ((e = launch_finalize(FK_BICG_W, B.st, B.partials, gr, 3, nullptr, s))) return e;
+      hipLaunchKernelGGL(k_bicg_xr, dim3(gr), dim3(kBlock), 0, s, B.st, n, (double2*)x, (const double2*)B.p.data,
+                         (const double2*)B.s.data, (const double2*)B.t.data, (const double2*)B.rh.data,
+                         (double2*)B.r.data, B.partials);
+      LYM_HIP(hipGetLastError());
+      if ((e = launch_finalize(FK_BICG_R, B.st, B.partials, gr, 3, nullptr, s))) return e;
+    }
+    return LYM_OK;
+  };
+  KeyBuf key;
+  key.add('B').add(square).add(shift).add(B.st).add(x).add(b);
+  rc = run_chunks(B.st, (int64_t)max_steps / per_chunk + 8, st, op_key(d) + key.s, chunk,
+                  [](const int*) { return (int)LYM_OK; });
+  if (rc) return rc;
+  LYM_HIP(hipMemcpyAsync(&h, B.st, sizeof(SolverState), hipMemcpyDeviceToHost, st));
+  LYM_HIP(hipStreamSynchronize(st));
+  fill_stats(stats, h, 1);
+  if (stats) stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return h.status == ST_CONVERGED ? LYM_OK : (h.status == ST_BREAKDOWN ? LYM_BREAKDOWN : LYM_NOT_CONVERGED);
 }
 
 int solve_mr(const OpDesc& d, int nshift, const double* shifts, int first, const void* b, void* const* xs,

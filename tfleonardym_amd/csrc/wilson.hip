@@ -96,6 +96,23 @@ struct EpiAxpy {
   }
 };
 
+// Store / axpy gated on a running solve (BiCGStab: launches enqueued after
+// the stop test cost only the launch).
+template <int P, int NCOMP>
+struct EpiGStore : EpiStore<P, NCOMP> {
+  SolverState* st;
+  EpiGStore(const EpiArgs& a, int64_t V) : EpiStore<P, NCOMP>(a, V), st(a.st) {}
+  __device__ EpiGStore at(int) const { return *this; }
+  __device__ bool active() const { return st->phase == PH_K1; }
+};
+template <int P, int NCOMP>
+struct EpiGAxpy : EpiAxpy<P, NCOMP> {
+  SolverState* st;
+  EpiGAxpy(const EpiArgs& a, int64_t V) : EpiAxpy<P, NCOMP>(a, V), st(a.st) {}
+  __device__ EpiGAxpy at(int) const { return *this; }
+  __device__ bool active() const { return st->phase == PH_K1; }
+};
+
 // CG step 1: t = Q p, reduce |t|^2 and |p|^2 (<p,(Q^2+s0)p> = |Qp|^2 + s0|p|^2).
 template <int P, int NCOMP>
 struct EpiCgmK1 {
@@ -883,6 +900,14 @@ int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, 
       if (kind == EPI_RU1) return launch_geom<P64, EpiRu1>(d, proj, in, a, rc, stream, nrhs);
       if (kind == EPI_RU2H) return launch_geom<P64, EpiRu2H>(d, proj, in, a, rc, stream, nrhs);
       return launch_geom<P64, EpiRu2>(d, proj, in, a, rc, stream, nrhs);
+    case EPI_GSTORE:
+    case EPI_GAXPY:
+      if (prec != P64 || nrhs != 1) {
+        set_error("BiCGStab runs in complex128, one RHS per call");
+        return LYM_ERR_UNSUPPORTED;
+      }
+      if (kind == EPI_GSTORE) return launch_geom<P64, EpiGStore>(d, proj, in, a, rc, stream, 1);
+      return launch_geom<P64, EpiGAxpy>(d, proj, in, a, rc, stream, 1);
     case EPI_MR_K1:
     case EPI_MR_K2:
       if (prec != P64 || nrhs != 1) {

@@ -234,9 +234,33 @@ class multishift_solver:
         self.last_status = "converged" if not active.any() else "max_iterations"
 
 
+def _wilson_form(operator):
+    """(W, square, shift) when operator is W, SquareOperator(W) or either
+    shifted by a real number, W a DiracWilsonOperator; else None."""
+    shift = 0.0
+    op = operator
+    if isinstance(op, dirac.ShiftedOperator):
+        sh = complex(op.shift)
+        if sh.imag != 0.0:
+            return None
+        shift = sh.real
+        op = op.operator_to_shift
+    w = dirac.wilson_of_square(op)
+    if w is not None:
+        return w, 1, shift
+    if isinstance(op, dirac.DiracWilsonOperator):
+        return op, 0, shift
+    return None
+
+
 class biconjugate_gradient:
     """BiCGStab as in dirac_solver.py:56-110 (shadow vector r_hat = x0 + A x0,
-    stop on |<r,r>| < precision), driven from Python on device vectors."""
+    stop on |<r,r>| < precision).  On a Wilson operator, its square or either
+    shifted (the reference's uses: Propagator, dirac.py:670-672, and the
+    negative pole of the multishift solver, dirac_solver.py:26-30) the whole
+    solve runs in liblym (lym_bicgstab: device-resident scalars, no host round
+    trip per iteration); any other operator runs the same recurrence from
+    Python on device vectors.  ``last_steps`` is the reference's step index."""
 
     def __init__(self, max_steps, precision, verbose=True):
         self.max_steps = max_steps
@@ -244,9 +268,13 @@ class biconjugate_gradient:
         self.verbose = verbose
         self.last_status = None
         self.last_steps = None
+        self.last_stats = None
 
     def solve(self, operator, source, initial_guess=None):
         b = to_device(source)
+        form = _wilson_form(operator)
+        if form is not None and b.dtype == torch.complex128:
+            return self._solve_device(form, b, initial_guess)
         x = b.clone() if initial_guess is None else to_device(initial_guess).clone()
         init = operator.apply(x)
         r = algebra_utils.lincomb(1.0, b, -1.0, init)
@@ -277,6 +305,32 @@ class biconjugate_gradient:
         self.last_status, self.last_steps = "max_iterations", step
         if self.verbose:
             print(f"BiconjugateGradient: failed to find convergence after {step} steps")
+        return x
+
+    def _solve_device(self, form, b, initial_guess):
+        w, square, shift = form
+        lib = _lib.lib()
+        desc = w.desc
+        x0 = None if initial_guess is None else to_device(initial_guess).to(torch.complex128).contiguous()
+        nbytes = lib.lym_bicgstab_workspace_bytes(ctypes.byref(desc))
+        if nbytes == 0:
+            raise RuntimeError(lib.lym_last_error().decode())
+        ws = _workspace(nbytes, b.device)
+        x = torch.empty_like(b)
+        st = _lib.LymSolveStats()
+        rc = _lib.check(lib.lym_bicgstab(ctypes.byref(desc), int(square), float(shift), _lib.ptr(b),
+                                         _lib.ptr(x0) if x0 is not None else None, _lib.ptr(x), int(self.max_steps),
+                                         float(self.precision), _lib.ptr(ws), ws.numel(), ctypes.byref(st),
+                                         _lib.stream_handle(b.device)), "bicgstab")
+        self.last_stats = st.as_dict(1)
+        step = max(0, int(st.iterations) - 1)
+        self.last_steps = step
+        self.last_status = _status_name(rc)
+        if self.verbose:
+            if rc == 0:
+                print(f"BiconjugateGradient: convergence after {step} steps")
+            else:
+                print(f"BiconjugateGradient: failed to find convergence after {step} steps")
         return x
 
 
