@@ -4,7 +4,7 @@ The reference file is parsed as text with ``ast`` (literal lists only); it is
 never imported or executed.  Run in the build container, where the
 reference is available:
 
-    python tfleonardym_amd/data/extract_tables.py /root/reference/approximations.py
+    python tests/golden/extract_tables.py /root/reference/approximations.py
 """
 import ast
 import json
@@ -34,7 +34,8 @@ def main(path):
                     ret = stmt.body[0]
                     pairs = literal_list(ret.value)
                     out["square_root_polynomial"][str(n)] = [[float(a), float(b)] for a, b in pairs]
-    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rational_tables.json")
+    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "..", "tfleonardym_amd", "data",
+                       "rational_tables.json")
     with open(dst, "w") as f:
         json.dump(out, f, indent=1)
     print(dst, {k: len(v) for k, v in out["rational"].items()}, list(out["square_root_polynomial"]))

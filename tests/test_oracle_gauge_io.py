@@ -61,3 +61,43 @@ def test_float_decoding_matches_xdrlib(tmp_path):
     ref = p.unpack_farray(4, p.unpack_float)
     assert list(og.read_words(data, 4)) == ref
     assert [p.unpack_int() for _ in range(2)] == [4, 6]
+
+
+def test_word_order_against_an_independent_writer(tmp_path):
+    """The U* file layout pinned by a writer that shares no code with the
+    oracle's: explicit loops over (site, mu, row, col < 2) emitting the
+    big-endian floats re(conj U[mu][row][col]), im(...) one by one with
+    struct.pack, as lattice.py:10-70 reads them (row index before column,
+    the conjugate stored).  A row/column transposition or a missing
+    conjugation in the oracle's reader fails here."""
+    geometry = (2, 3)
+    V = 6
+    U = fields.gauge_field(geometry, 3, seed=21, kind="haar")
+    path = tmp_path / "loop.xdr"
+    with open(path, "wb") as f:
+        for site in range(V):
+            for mu in range(2):
+                for row in range(3):
+                    for col in range(2):
+                        z = np.conj(U[mu, row, col, site])
+                        f.write(struct.pack(">f", float(z.real)))
+                        f.write(struct.pack(">f", float(z.imag)))
+        f.write(struct.pack(">2i", *geometry))
+        f.write(struct.pack(">3d", 1.0, 2.0, 3.0))
+    got, dims, params = og.read_config(path, geometry)
+    assert dims == list(geometry) and params == [1.0, 2.0, 3.0]
+    assert np.abs(got - U).max() < 3e-7
+    # a transposed file (column index before row) is a different field
+    path2 = tmp_path / "transposed.xdr"
+    with open(path2, "wb") as f:
+        for site in range(V):
+            for mu in range(2):
+                for col in range(2):
+                    for row in range(3):
+                        z = np.conj(U[mu, row, col, site])
+                        f.write(struct.pack(">f", float(z.real)))
+                        f.write(struct.pack(">f", float(z.imag)))
+        f.write(struct.pack(">2i", *geometry))
+        f.write(struct.pack(">3d", 1.0, 2.0, 3.0))
+    bad, _, _ = og.read_config(path2, geometry)
+    assert np.abs(bad - U).max() > 1e-2

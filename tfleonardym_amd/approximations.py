@@ -1,7 +1,7 @@
 """Rational and polynomial approximation tables (approximations.py:4-139).
 
 The coefficient tables are data, loaded from ``data/rational_tables.json``
-(extracted from the reference's literal lists by data/extract_tables.py).
+(extracted from the reference's literal lists by tests/golden/extract_tables.py).
 Each rational table is the concatenation alpha || beta; ``split`` cuts it at
 len//2 as the reference's driver does (TfLeonardYMNDim.py:310-332).
 """
