@@ -122,3 +122,22 @@ def test_half_storage_hop_matches_oracle(lib, geometry, nc, herm, bc, proj):
     assert rc == 0
     err = np.linalg.norm(out.astype(np.complex128) - want) / np.linalg.norm(want)
     assert err < 2e-6, err
+
+
+def test_buffer_range_predicate(lib):
+    """The host check of buffer arguments (check_buffers in liblym): a buffer
+    of `bytes` starting at p must end inside the allocation [base, base+size)
+    that contains p.  Exercised on the host build; a short buffer is never
+    launched on the GPU."""
+    f = lib.cpucheck_range_inside
+    f.argtypes = [ctypes.c_uint64] * 4
+    f.restype = ctypes.c_int
+    base, size = 0x7F0000000000, 4096
+    assert f(base, size, base, 4096) == 1          # exactly the allocation
+    assert f(base, size, base + 1024, 3072) == 1   # a tail slice
+    assert f(base, size, base + 1024, 3073) == 0   # one byte past the end
+    assert f(base, size, base, 4097) == 0
+    assert f(base, size, base - 8, 16) == 0        # starts before the allocation
+    assert f(base, size, base + 4096, 1) == 0      # starts at the end
+    assert f(base, size, base + 4096, 0) == 1      # empty range at the end
+    assert f(base, size, base + 8, 2**64 - 8) == 0  # no wrap-around

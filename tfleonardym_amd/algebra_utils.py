@@ -26,7 +26,10 @@ def dot(v1, v2):
 
 
 def axpby(a, x, b, y):
-    """y <- a x + b y in place (complex scalars), on the device."""
+    """y <- a x + b y in place (complex scalars), on the device.  x and y
+    must be distinct tensors (the kernel takes them as non-aliasing)."""
+    if x.data_ptr() == y.data_ptr():
+        raise RuntimeError("axpby: x and y must not alias")
     a = complex(a)
     b = complex(b)
     _lib.check(_lib.lib().lym_axpby(_lib.PREC_F64, y.numel(), a.real, a.imag, _lib.ptr(x), b.real, b.imag,

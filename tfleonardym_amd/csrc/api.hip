@@ -28,6 +28,48 @@ int hip_fail(hipError_t e, const char* what) {
 
 int64_t field_elems(const OpDesc& d) { return (int64_t)d.ncomp * d.lat.V; }
 
+static int check_one(const void* p, size_t bytes, const char* what) {
+  if (!p || bytes == 0) return LYM_OK;
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  const hipError_t e = hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // not a device allocation the runtime tracks: cannot check
+    return LYM_OK;
+  }
+  if (!range_inside((uintptr_t)base, size, (uintptr_t)p, bytes)) {
+    set_error(std::string(what) + ": buffer too small (the call reads or writes " + std::to_string(bytes) +
+              " bytes, the allocation holds " + std::to_string((uintptr_t)base + size - (uintptr_t)p) +
+              " from this pointer)");
+    return LYM_ERR_ARG;
+  }
+  return LYM_OK;
+}
+
+int check_buffers(const OpDesc* d, hipStream_t st, std::initializer_list<Buf> bufs) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return LYM_OK;
+  (void)hipGetLastError();
+  if (d) {
+    const size_t lb = (size_t)d->ndim * d->nc * d->nc * d->lat.V;  // complex numbers per link field
+    int rc = check_one(d->links64, lb * 16, "links");
+    if (!rc) rc = check_one(d->linksr64, lb * 16, "links_right");
+    if (!rc) rc = check_one(d->links32, lb * 8, "links_f32");
+    if (!rc) rc = check_one(d->linksr32, lb * 8, "links_right_f32");
+    if (rc) return rc;
+  }
+  for (const Buf& b : bufs) {
+    const int rc = check_one(b.p, b.bytes, b.what);
+    if (rc) return rc;
+  }
+  return LYM_OK;
+}
+
+// bytes of one field of the operator's layout at precision prec
+static size_t fbytes(const OpDesc& d, int prec) {
+  return (size_t)field_elems(d) * (prec == LYM_PREC_F64 ? 16 : prec == LYM_PREC_F32 ? 8 : 4);
+}
+
 // Every LDS tile a 4D fundamental lattice admits (L0 divisible by the XCD
 // count): a0 | L0/8, a1 | L1, k | L2, a0*a1*k*L3 a multiple of 64 and at
 // most max_threads sites, lds_per_site * sites within max_lds bytes.
@@ -210,7 +252,15 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
         o.tile_tuned[pi] = 1;
       }
     }
-    if (want_hif == 2 || want_hif == 4) best.hif = want_hif;
+    if (want_hif == 2 || want_hif == 4) {
+      // the launcher's limits (wilson.hip): hops in flight need blocks of at
+      // most tile_hif_threads sites, and 4 is built for complex64 only; the
+      // recorded value is the one that runs
+      int h = want_hif;
+      if (pi == 0 && h == 4) h = 2;
+      if (best.nsite > tile_hif_threads(pi == 0 ? P64 : P32)) h = 1;
+      best.hif = h;
+    }
     o.tile[pi] = best;
   }
   // multi-RHS tile (complex64): kMrRhs spinors per site in LDS.  Opt-in
@@ -264,6 +314,9 @@ int lym_wilson_apply(const lym_wilson* op, int prec, int mode, const void* in, v
   OpDesc d;
   int rc = make_desc(op, &d);
   if (rc) return rc;
+  if ((rc = check_buffers(&d, (hipStream_t)stream, {{in, fbytes(d, prec), "lym_wilson_apply in"},
+                                                     {out, fbytes(d, prec), "lym_wilson_apply out"}})))
+    return rc;
   if (!in || !out || in == out) {
     set_error("lym_wilson_apply: in/out must be distinct non-null device pointers");
     return LYM_ERR_ARG;
@@ -293,6 +346,10 @@ int lym_wilson_apply_square(const lym_wilson* op, int prec, double shift, const 
     set_error("lym_wilson_apply_square: precision must be F64 or F32");
     return LYM_ERR_ARG;
   }
+  if ((rc = check_buffers(&d, (hipStream_t)stream, {{in, fbytes(d, prec), "lym_wilson_apply_square in"},
+                                                     {out, fbytes(d, prec), "lym_wilson_apply_square out"},
+                                                     {tmp, fbytes(d, prec), "lym_wilson_apply_square tmp"}})))
+    return rc;
   EpiArgs a;
   a.out.data = tmp;
   Field f;
@@ -468,6 +525,14 @@ int lym_lie_derivative(const lym_wilson* op, int npairs, const void* const* xs, 
       return LYM_ERR_ARG;
     }
   }
+  {
+    const size_t ob = (size_t)d.ndim * d.nc * d.nc * d.lat.V * 16;  // (D, nc, nc, V) complex128
+    if ((rc = check_buffers(&d, (hipStream_t)stream, {{out, ob, "lym_lie_derivative out"}}))) return rc;
+    for (int p = 0; p < npairs; ++p)
+      if ((rc = check_buffers(nullptr, (hipStream_t)stream, {{xs[p], fbytes(d, LYM_PREC_F64), "lym_lie_derivative xs"},
+                                                              {ys[p], fbytes(d, LYM_PREC_F64), "lym_lie_derivative ys"}})))
+        return rc;
+  }
   if (npairs == 0 && accumulate) return LYM_OK;
   // launches of at most kForceMaxPairs pairs; the later ones accumulate
   int p0 = 0;
@@ -600,6 +665,12 @@ int lym_mr_multishift(const lym_wilson* op, int nshift, const double* shifts, in
     set_error("lym_mr_multishift: null argument");
     return LYM_ERR_ARG;
   }
+  if ((rc = check_buffers(&d, (hipStream_t)stream, {{b, fbytes(d, LYM_PREC_F64), "lym_mr_multishift b"},
+                                                     {workspace, workspace_bytes, "lym_mr_multishift workspace"}})))
+    return rc;
+  for (int k = first; k >= 0 && k < nshift && k < 64; ++k)
+    if ((rc = check_buffers(nullptr, (hipStream_t)stream, {{xs[k], fbytes(d, LYM_PREC_F64), "lym_mr_multishift xs"}})))
+      return rc;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   return solve_mr(d, nshift, shifts, first, b, xs, max_steps, precision, omega, workspace, workspace_bytes, stats,
                   (hipStream_t)stream);
@@ -613,6 +684,14 @@ int lym_cgm(const lym_wilson* op, int prec, int nshift, const double* shifts, co
   if (!b || !xs || !shifts || !(tol > 0.0) || max_iter < 1) {
     set_error("lym_cgm: bad arguments");
     return LYM_ERR_ARG;
+  }
+  if (nshift >= 1 && nshift <= 64) {
+    if ((rc = check_buffers(&d, (hipStream_t)stream, {{b, fbytes(d, LYM_PREC_F64), "lym_cgm b"},
+                                                       {workspace, workspace_bytes, "lym_cgm workspace"}})))
+      return rc;
+    for (int k = 0; k < nshift; ++k)
+      if ((rc = check_buffers(nullptr, (hipStream_t)stream, {{xs[k], fbytes(d, LYM_PREC_F64), "lym_cgm xs[k]"}})))
+        return rc;
   }
   if (stats) std::memset(stats, 0, sizeof(*stats));
   return solve_cgm(d, prec, nshift, shifts, b, xs, tol, max_iter, workspace, workspace_bytes, stats,
@@ -628,6 +707,18 @@ int lym_cgm_batched(const lym_wilson* op, int prec, int nrhs, int nshift, const 
   if (!bs || !xs || !shifts || !(tol > 0.0) || max_iter < 1 || nrhs < 1) {
     set_error("lym_cgm_batched: bad arguments");
     return LYM_ERR_ARG;
+  }
+  if (nshift >= 1 && nshift <= 64) {
+    if ((rc = check_buffers(&d, (hipStream_t)stream, {{workspace, workspace_bytes, "lym_cgm_batched workspace"}})))
+      return rc;
+    for (int r = 0; r < nrhs; ++r) {
+      if ((rc = check_buffers(nullptr, (hipStream_t)stream, {{bs[r], fbytes(d, LYM_PREC_F64), "lym_cgm_batched bs[r]"}})))
+        return rc;
+      for (int k = 0; k < nshift; ++k)
+        if ((rc = check_buffers(nullptr, (hipStream_t)stream,
+                                {{xs[(size_t)r * nshift + k], fbytes(d, LYM_PREC_F64), "lym_cgm_batched xs"}})))
+          return rc;
+    }
   }
   if (stats) std::memset(stats, 0, sizeof(*stats) * nrhs);
   return solve_cgm_batched(d, prec, nrhs, nshift, shifts, bs, xs, tol, max_iter, workspace, workspace_bytes, stats,
@@ -650,6 +741,11 @@ int lym_bicgstab(const lym_wilson* op, int square, double shift, const void* b, 
     set_error("lym_bicgstab: bad arguments");
     return LYM_ERR_ARG;
   }
+  if ((rc = check_buffers(&d, (hipStream_t)stream, {{b, fbytes(d, LYM_PREC_F64), "lym_bicgstab b"},
+                                                     {x0, fbytes(d, LYM_PREC_F64), "lym_bicgstab x0"},
+                                                     {x, fbytes(d, LYM_PREC_F64), "lym_bicgstab x"},
+                                                     {workspace, workspace_bytes, "lym_bicgstab workspace"}})))
+    return rc;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   return solve_bicgstab(d, square, shift, b, x0, x, max_steps, precision, workspace, workspace_bytes, stats,
                         (hipStream_t)stream);

@@ -180,3 +180,8 @@ extern "C" int lymcheck_wilson_apply_half(int ndim, int nc, int herm, const int*
   }
   return -1;
 }
+
+// the buffer-range predicate of the host argument checks (lym_types.h)
+extern "C" int cpucheck_range_inside(uint64_t base, uint64_t size, uint64_t p, uint64_t bytes) {
+  return range_inside((uintptr_t)base, (size_t)size, (uintptr_t)p, (size_t)bytes) ? 1 : 0;
+}

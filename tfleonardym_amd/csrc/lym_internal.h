@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <atomic>
+#include <initializer_list>
 #include <string>
 #include <vector>
 
@@ -80,6 +81,19 @@ inline int raise_lds_limit(const void* fn, int bytes, std::atomic<uint32_t>& don
 }
 
 int make_desc(const lym_wilson* w, OpDesc* d);
+
+// A device buffer argument and the bytes the call will touch from it.
+struct Buf {
+  const void* p;
+  size_t bytes;
+  const char* what;
+};
+// Host-side check that every buffer (and the operator's links) lies inside
+// the device allocation that contains its start, as the runtime reports it
+// (hipMemGetAddressRange): a short buffer returns LYM_ERR_ARG instead of being
+// read past its end.  Skipped while `st` is being captured into a graph, and
+// for pointers the runtime does not know.
+int check_buffers(const OpDesc* d, hipStream_t st, std::initializer_list<Buf> bufs);
 int64_t field_elems(const OpDesc& d);  // complex numbers per spinor field
 
 // ---- Wilson kernel launchers (wilson.hip) --------------------------------

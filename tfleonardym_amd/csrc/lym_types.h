@@ -335,6 +335,12 @@ struct LinkView {
   }
 };
 
+// [p, p + bytes) lies inside the allocation [base, base + size) (the host
+// check of buffer arguments against their allocations)
+__host__ __device__ inline bool range_inside(uintptr_t base, size_t size, uintptr_t p, size_t bytes) {
+  return p >= base && p - base <= size && bytes <= size - (p - base);
+}
+
 // Lattice description passed by value to every Wilson kernel.
 struct Lat {
   int L[4];

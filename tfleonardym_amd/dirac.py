@@ -224,7 +224,11 @@ class Overlap(Operator):
         force = None
         for (a, b), w in zip(pairs, weights):
             term = D.lie_derivative(a, b)
-            force = algebra_utils.axpby(w, term, 0.0, term) if force is None else algebra_utils.axpby(w, term, 1.0, force)
+            if force is None:
+                # x and y of lym_axpby must not alias: scale into a fresh tensor
+                force = algebra_utils.axpby(w, term, 0.0, torch.zeros_like(term))
+            else:
+                force = algebra_utils.axpby(w, term, 1.0, force)
         return force
 
 
