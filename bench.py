@@ -47,6 +47,10 @@ BATCHED = {
                 name="wilson_ddag_d_24c3x48_su3_batched28"),
     "c2b": dict(geometry=(16, 16, 16, 32), nc=3, kappa=0.13, eps=0.3, npoint=12, nz2=16, precision="half",
                 name="wilson_ddag_d_16c3x32_su3_batched28"),
+    # batched multishift: 4 point + 4 Z2 sources, 12 shifts log-spaced in
+    # [1e-4, 10], every (source, shift) to 1e-10 (16-bit storage phases)
+    "c3m": dict(geometry=(24, 24, 24, 48), nc=3, kappa=0.1335, eps=0.25, npoint=4, nz2=4, precision="half",
+                nshift=12, lo=1e-4, hi=10.0, name="wilson_ddag_d_24c3x48_su3_batched8_multishift12"),
 }
 
 SERVING = {
@@ -139,7 +143,9 @@ def batched_result(workload, precision, steps, warmup, baseline_steps, autotune=
     geometry, nc = W["geometry"], W["nc"]
     U = fields.gauge_field(geometry, nc, seed=20250, kind="near_identity", eps=W["eps"])
     srcs = [fields.point_source(geometry, 4, nc, site=0, color=c, spin=s) for s in range(4) for c in range(nc)]
-    srcs = srcs[:W["npoint"]] + [fields.z2_noise(geometry, 4, nc, seed=20260 + i) for i in range(W["nz2"])]
+    srcs = srcs[::max(1, len(srcs) // W["npoint"])][:W["npoint"]]
+    srcs = srcs + [fields.z2_noise(geometry, 4, nc, seed=20260 + i) for i in range(W["nz2"])]
+    shifts = fields.log_shifts(W["lo"], W["hi"], W["nshift"]) if "nshift" in W else [0.0]
     srcs = [torch.from_numpy(s).to(dev) for s in srcs]
     op = dirac.DiracWilsonOperator(lattice.Configuration(U, geometry, nc, 4), W["kappa"], True,
                                    representation="fundamental")
@@ -149,11 +155,11 @@ def batched_result(workload, precision, steps, warmup, baseline_steps, autotune=
     solver = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode=mode,
                                             verbose=False)
     for _ in range(warmup):
-        solver.solve_batch(A, srcs, [0.0])
+        solver.solve_batch(A, srcs, shifts)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(steps):
-        solver.solve_batch(A, srcs, [0.0])
+        solver.solve_batch(A, srcs, shifts)
         st = solver.last_stats
         worst = max(max(s["true_rel_resid"]) for s in st)
         print(f"[bench] batched {workload} step {i}: {st[0]['seconds'] * 1e3:.0f} ms, {st[0]['iterations']} it, "
@@ -170,22 +176,23 @@ def batched_result(workload, precision, steps, warmup, baseline_steps, autotune=
                                             verbose=False)
         nb = min(len(srcs), max(2, baseline_steps))
         pick = sorted(set(np.linspace(0, len(srcs) - 1, nb).round().astype(int).tolist()))
-        ps.solve(A, srcs[0], [0.0])
+        ps.solve(A, srcs[0], shifts)
         torch.cuda.synchronize()
         tb = time.perf_counter()
         for s in [srcs[i] for i in pick]:
-            ps.solve(A, s, [0.0])
+            ps.solve(A, s, shifts)
         torch.cuda.synchronize()
         plain = len(pick) / (time.perf_counter() - tb)
         print(f"[bench] plain FP64 CG, one source at a time: {plain:.2f} solves/s", file=sys.stderr, flush=True)
     return {
-        "metric": "batched multi-RHS CG solves/s at 1e-10 true resid",
+        "metric": ("batched multi-RHS CG solves/s at 1e-10 true resid" if len(shifts) == 1 else
+                   "batched multishift CG solves/s (one solve = one source, all shifts) at 1e-10 true resid"),
         "value": value, "unit": "solves/s", "steps": steps, "warmup": warmup,
         "ms_per_step": total / steps * 1e3,
         "dtype": {"half": "f16 storage + f32 arithmetic + f64 reliable updates", "mixed": "f32+f64",
                   "double": "f64"}[mode],
         "config": {"workload": W["name"], "geometry": list(geometry), "kappa": W["kappa"], "rhs": len(srcs),
-                   "precision": mode, "tolerance": 1e-10},
+                   "shifts": len(shifts), "precision": mode, "tolerance": 1e-10},
         "iterations": solver.last_stats[0]["iterations"],
         "plain_fp64_single_rhs_solves_per_s": plain,
         "speedup_vs_plain_fp64": (value / plain) if plain else None,
