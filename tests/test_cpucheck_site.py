@@ -89,6 +89,15 @@ def encode16(psi):
     return ints, norm
 
 
+def pack16(ints):
+    """(NCOMP, V, 2) int16 -> the kernels' packed layout: G complex numbers per
+    element (G = 4 if NCOMP % 4 == 0 else 2), component c of site s at
+    element (c // G) * V + s, slot c % G."""
+    ncomp, V, _ = ints.shape
+    g = 4 if ncomp % 4 == 0 else 2
+    return np.ascontiguousarray(ints.reshape(ncomp // g, g, V, 2).transpose(0, 2, 1, 3))
+
+
 @pytest.mark.parametrize("geometry,nc", [((4, 6), 2), ((3, 4), 3), ((2, 4, 2, 4), 2), ((4, 2, 3, 4), 3)])
 @pytest.mark.parametrize("herm", [0, 1])
 @pytest.mark.parametrize("bc", [None, -1.0])
@@ -117,7 +126,7 @@ def test_half_storage_hop_matches_oracle(lib, geometry, nc, herm, bc, proj):
     b = 1.0 + 0j if bc is None else complex(bc)
     rc = lib.lymcheck_wilson_apply_half(ndim, nc, herm, dims, ctypes.c_double(kappa), ctypes.c_double(b.real),
                                         ctypes.c_double(b.imag), proj, links.ctypes.data_as(ctypes.c_void_p),
-                                        np.ascontiguousarray(ints).ctypes.data_as(ctypes.c_void_p),
+                                        pack16(ints).ctypes.data_as(ctypes.c_void_p),
                                         norm.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p))
     assert rc == 0
     err = np.linalg.norm(out.astype(np.complex128) - want) / np.linalg.norm(want)

@@ -141,9 +141,15 @@ __device__ __forceinline__ void buf_st(T* base, uint32_t bytes, int64_t plane, i
   const int voff = (int)((uint32_t)s * (uint32_t)sizeof(T));
   const int soff = (int)((uint32_t)(plane * V) * (uint32_t)sizeof(T));
   if constexpr (sizeof(T) == 16) {
+    // 16-byte stores take the whole offset in the VGPR and soffset = 0.
+    // With an SGPR soffset the compiler inserts no wait state between the
+    // store and a VALU write of its data registers, and on gfx950 such a
+    // write right after buffer_store_dwordx4 was measured to corrupt the
+    // stored value (16-bit packed stores from 512-thread blocks, two waves
+    // per SIMD); with soffset = 0 the compiler's hazard guard applies.
     __attribute__((ext_vector_type(4))) unsigned int v;
     __builtin_memcpy(&v, &val, 16);
-    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff + soff, 0, 0);
   } else if constexpr (sizeof(T) == 8) {
     __attribute__((ext_vector_type(2))) unsigned int v;
     __builtin_memcpy(&v, &val, 8);
@@ -178,12 +184,30 @@ __host__ __device__ __forceinline__ void put(T* base, uint32_t bytes, int64_t pl
 
 // ---------------------------------------------------------------------------
 // Storage of spinor fields: component planes, site index fastest.
-//   component c of site s lives at  base[c * V + s]
-// P64: complex128 (interleaved re/im), P32: complex64, P16: 16-bit fixed
-// point scaled per site (value = int16 * norm[s], norm[s] = max |component|
-// / 32767): every component of a site keeps 15 bits below the site maximum,
-// 16x finer than IEEE half for the same 4 bytes per complex number.
+//   P64 / P32: component c of site s lives at  base[c * V + s]
+//   (complex128 / complex64, interleaved re/im).
+// P16: 16-bit fixed point scaled per site (value = int16 * norm[s],
+// norm[s] = max |component| / 32767): every component of a site keeps 15
+// bits below the site maximum, 16x finer than IEEE half for the same 4 bytes
+// per complex number.  The integers are packed G complex numbers per
+// 4G-byte element (G = 4 when NCOMP is a multiple of 4, else 2): component c
+// of site s is short2 number (c / G * V + s) * G + c % G, so one lane reads
+// a site's NCOMP components in NCOMP / G loads of 16 (8) bytes and a wave
+// reads 1 KiB (512 B) contiguous per load.  The layout is internal to the
+// solvers (their Krylov vectors); the caller's fields are complex128.
 // ---------------------------------------------------------------------------
+template <int NCOMP>
+struct Pack16 {
+  static constexpr int G = (NCOMP % 4 == 0) ? 4 : 2;
+  struct alignas(4 * G) unit {
+    short v[2 * G];
+  };
+};
+// short2 index of component c of site s (packed 16-bit layout)
+__host__ __device__ __forceinline__ int64_t p16_index(int ncomp, int c, int64_t V, int64_t s) {
+  const int g = (ncomp % 4 == 0) ? 4 : 2;
+  return ((int64_t)(c / g) * V + s) * g + c % g;
+}
 template <int P>
 struct Store;
 
@@ -223,11 +247,7 @@ struct SpinorView {
     const uint32_t bytes = (uint32_t)(NCOMP * V * (int64_t)sizeof(E));
     if constexpr (P == P16) {
       const float n = ld<float>(norm, (uint32_t)(V * 4), 0, V, s);
-#pragma unroll
-      for (int c = 0; c < NCOMP; ++c) {
-        const E e = ld<E>(data, bytes, c, V, s);
-        v[c] = mk<R>((float)e.x * n, (float)e.y * n);
-      }
+      load_units<NCOMP>(s, v, n);
     } else {
 #pragma unroll
       for (int c = 0; c < NCOMP; ++c) {
@@ -246,16 +266,24 @@ struct SpinorView {
   // its norm apart (value = v * n) and apply n once to the transported
   // half-spinor, inside the accumulating FMAs (Wilson hops are linear in v)
   static constexpr bool kScaled = (P == P16);
+  // the packed integers of site s as floats, each scaled by sc
+  template <int NCOMP>
+  __host__ __device__ __forceinline__ void load_units(int64_t s, cpx<R> (&v)[NCOMP], float sc) const {
+    using U = typename Pack16<NCOMP>::unit;
+    constexpr int G = Pack16<NCOMP>::G;
+    const uint32_t bytes = (uint32_t)(NCOMP * V * (int64_t)sizeof(E));
+#pragma unroll
+    for (int q = 0; q < NCOMP / G; ++q) {
+      const U u = ld<U>((const U*)data, bytes, q, V, s);
+#pragma unroll
+      for (int g = 0; g < G; ++g) v[q * G + g] = mk<R>((float)u.v[2 * g] * sc, (float)u.v[2 * g + 1] * sc);
+    }
+  }
   template <int NCOMP>
   __host__ __device__ __forceinline__ void load_raw(int64_t s, cpx<R> (&v)[NCOMP], R& n) const {
     if constexpr (P == P16) {
-      const uint32_t bytes = (uint32_t)(NCOMP * V * (int64_t)sizeof(E));
       n = ld<float>(norm, (uint32_t)(V * 4), 0, V, s);
-#pragma unroll
-      for (int c = 0; c < NCOMP; ++c) {
-        const E e = ld<E>(data, bytes, c, V, s);
-        v[c] = mk<R>((float)e.x, (float)e.y);
-      }
+      load_units<NCOMP>(s, v, 1.0f);
     } else {
       n = R(1);
       load<NCOMP>(s, v);
@@ -278,12 +306,17 @@ struct SpinorView {
       const float scale = m * (1.0f / 32767.0f);
       const float inv = m > 0.f ? 32767.0f / m : 0.f;
       put<float>(norm, (uint32_t)(V * 4), 0, V, s, scale);
+      using U = typename Pack16<NCOMP>::unit;
+      constexpr int G = Pack16<NCOMP>::G;
 #pragma unroll
-      for (int c = 0; c < NCOMP; ++c) {
-        E e;
-        e.x = (short)fminf(fmaxf(rintf(v[c].re * inv), -32767.f), 32767.f);
-        e.y = (short)fminf(fmaxf(rintf(v[c].im * inv), -32767.f), 32767.f);
-        put<E>(data, bytes, c, V, s, e);
+      for (int q = 0; q < NCOMP / G; ++q) {
+        U u;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          u.v[2 * g] = (short)fminf(fmaxf(rintf(v[q * G + g].re * inv), -32767.f), 32767.f);
+          u.v[2 * g + 1] = (short)fminf(fmaxf(rintf(v[q * G + g].im * inv), -32767.f), 32767.f);
+        }
+        put<U>((U*)data, bytes, q, V, s, u);
       }
     } else {
 #pragma unroll

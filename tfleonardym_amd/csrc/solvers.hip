@@ -418,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int n
       ri[j] = RR(0);
       if (j < nb) {
         const int q = (k0 + j + 1) % nslot;
-        const E e = rg[(int64_t)q * n + i];
+        const E e = rg[(int64_t)q * n + (PR == P16 ? p16_index(ncomp, c, V, site) : i)];
         if constexpr (PR == P16) {
           const float sc = rn[(int64_t)q * V + site];
           rr[j] = (float)e.x * sc;
