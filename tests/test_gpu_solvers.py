@@ -247,6 +247,32 @@ def test_batch_tuned_tile(mode, monkeypatch):
         assert rel(xt, xf.cpu().numpy()) < 1e-8
 
 
+@pytest.mark.parametrize("tile", ["1,8,4", "2,4,4", "1,4,4"])
+def test_half_cg_tile_sizes(tile, monkeypatch):
+    """16-bit reliable-update CG on 512-site (two waves per SIMD) and 256-site
+    LDS tiles: the iteration count of the complex64 solve within 2% and the
+    oracle-recomputed residual at 1e-10.  512-site tiles once stalled near
+    1e-5 when a 16-byte store's data registers were overwritten right after
+    the store was issued."""
+    from tfleonardym_amd import dirac, dirac_solver
+    geometry = (8, 8, 4, 16)
+    print(f"case half CG tile={tile}", flush=True)
+    monkeypatch.setenv("LYM_TILE", tile)
+    monkeypatch.setenv("LYM_NO_BATCH_TUNE", "1")
+    op, oop = setup(geometry, 3, 0.13)
+    b = fields.gaussian_spinor(geometry, 4, 3, seed=90)
+    its = {}
+    for mode in ("half", "mixed"):
+        s = dirac_solver.multishift_solver(3000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode=mode,
+                                           verbose=False)
+        x = s.solve(dirac.SquareOperator(op), b, [0.0])[0]
+        assert s.last_status == "converged", (mode, s.last_stats)
+        assert osv.true_residual(oop.apply_square, b, x.cpu().numpy(), 0.0) <= 1e-10
+        its[mode] = s.last_stats["iterations"]
+    assert op.tile_info("f16")["nsite"] == int(np.prod([int(v) for v in tile.split(",")])) * geometry[3]
+    assert its["half"] <= 1.02 * its["mixed"] + 2, its
+
+
 @pytest.mark.parametrize("hif", ["1", "2"])
 def test_half_batch_hops_in_flight(hif, monkeypatch):
     """The 16-bit batched CG with one or two hops in flight per wave (forced
