@@ -12,12 +12,21 @@ residual `tolerance`.  Behaviour:
   fit yet waits while later, smaller ones may start.  A request waiting longer
   than `max_wait_s` blocks later admissions until it fits, so it cannot
   starve.
+* Requests with the same field, kappa, shift list and tolerance that wait
+  together are merged (up to `max_batch`) into one lym_cgm_batched call: one
+  launch sequence per iteration covers all of them, and the blocks of the
+  group's sources on one site tile run back to back on one XCD, so their link
+  reads share that XCD's L2.
 * Each slot is a worker thread with its own HIP stream.  ctypes releases the
   GIL during a solve, so solves of different requests overlap on the device,
-  and a slot takes new work as soon as its solve ends.
-* Every request has its own workspace and the kernels' reductions have a
-  fixed order, so a result is bitwise the same whether the request runs alone
-  or in a mix (tests/test_gpu_serving.py).
+  and a slot takes new work as soon as its solve ends.  A source that is
+  already a device tensor is read after the work queued before submit on the
+  caller's current stream (an event recorded at submit).
+* A request solved on its own has its own workspace, and the kernels'
+  reductions have a fixed order, so its result is bitwise the same whether it
+  runs alone or beside others (tests/test_gpu_serving.py).  A merged request
+  meets the same tolerance; its iterates follow the group's precision
+  switches, so its bits can differ from a solve on its own.
 
 Latency is measured from `arrival` (set at submit unless given) to the moment
 the result is ready on the host side (device synchronised).
@@ -120,18 +129,18 @@ class GaugeStore:
 class SolveServer:
     """Continuous-admission multishift solve server (see module docstring)."""
 
-    # "half": one-shift requests run the 16-bit-storage CG with FP64 reliable
-    # updates, multishift requests the mixed CG-M (16-bit storage is built
-    # for one shift only)
+    # "half": r, p and Q p in 16-bit storage with per-site norms (one shift:
+    # CG with FP64 reliable updates; several: the three-phase CG-M)
     _PREC = {"mixed": _lib.PREC_F32, "double": _lib.PREC_F64, "half": _lib.PREC_F16}
 
     def __init__(self, store, slots=4, hbm_budget_bytes=None, precision_mode="mixed", max_iter=50000,
-                 max_wait_s=2.0):
+                 max_wait_s=2.0, max_batch=8):
         self.store = store
         self.slots = int(slots)
         self.prec = self._PREC[precision_mode]
         self.max_iter = int(max_iter)
         self.max_wait_s = float(max_wait_s)
+        self.max_batch = max(1, int(max_batch))
         if hbm_budget_bytes is None:
             free, _ = torch.cuda.mem_get_info(store.device)
             hbm_budget_bytes = int(0.8 * free)
@@ -144,6 +153,7 @@ class SolveServer:
         self._next_id = 0
         self._threads = []
         self.max_inflight_seen = 0
+        self.batches = collections.Counter()  # group size -> launches
 
     # ---- lifecycle ---------------------------------------------------------
     def start(self):
@@ -169,18 +179,32 @@ class SolveServer:
 
     # ---- admission -----------------------------------------------------------
     def _prec_for(self, nshift):
-        """Storage precision of a request with `nshift` shifts."""
-        return self.prec if (self.prec != _lib.PREC_F16 or nshift == 1) else _lib.PREC_F32
+        """Storage precision of a request (the store's fields are fundamental,
+        for which every precision is built at every shift count)."""
+        return self.prec
 
-    def footprint(self, req):
-        """Device bytes a request holds while it runs."""
+    @staticmethod
+    def _group_key(req):
+        """Requests with equal keys share one operator and shift list and are
+        solved together in one lym_cgm_batched call."""
+        return (req.field, float(req.kappa), tuple(float(s) for s in req.shifts), float(req.tolerance))
+
+    def footprint(self, req, nrhs=1):
+        """Device bytes `nrhs` requests like `req` hold while they run together:
+        the solver workspace (lym_cgm_batched's for a group), solutions and
+        sources."""
         d = self.store.desc(req.field, req.kappa)
         n = len(req.shifts)
-        ws = int(_lib.lib().lym_solver_workspace_bytes(ctypes.byref(d), self._prec_for(n), n))
+        prec = self._prec_for(n)
+        lib = _lib.lib()
+        if nrhs == 1:
+            ws = int(lib.lym_solver_workspace_bytes(ctypes.byref(d), prec, n))
+        else:
+            ws = int(lib.lym_cgm_batched_workspace_bytes(ctypes.byref(d), prec, n, nrhs))
         if ws == 0:
-            raise RuntimeError(_lib.lib().lym_last_error().decode())
+            raise RuntimeError(lib.lym_last_error().decode())
         elems = int(np.prod(self.store.spinor_shape(req.field)))
-        return ws + (n + 1) * elems * 16
+        return ws + nrhs * (n + 1) * elems * 16
 
     def submit(self, req):
         fut = Future()
@@ -188,24 +212,48 @@ class SolveServer:
         if fp > self.budget:
             fut.set_exception(RuntimeError(f"request needs {fp} bytes, above the budget {self.budget}"))
             return fut
+        # a source already on the device may still be written by work queued on
+        # the caller's stream: the solve's stream waits for this event
+        ready = None
+        src = req.source
+        if isinstance(src, torch.Tensor) and src.is_cuda:
+            ready = torch.cuda.Event()
+            ready.record(torch.cuda.current_stream(src.device))
         with self._cv:
             if req.rid < 0:
                 req.rid = self._next_id
             self._next_id += 1
             if req.arrival is None:
                 req.arrival = time.perf_counter()
-            self._pending.append((req, fp, fut))
+            self._pending.append((req, fp, fut, ready))
             self._cv.notify_all()
         return fut
 
     def _take(self):
-        """Under the lock: first request (arrival order) that fits, or None."""
+        """Under the lock: the first request (arrival order) that fits, with up
+        to max_batch - 1 later requests of the same group key as far as the
+        group's footprint fits; or None."""
         now = time.perf_counter()
-        for i, (req, fp, fut) in enumerate(self._pending):
-            fits = self._inflight_bytes + fp <= self.budget
-            if fits:
-                del self._pending[i]
-                return req, fp, fut
+        for i, (req, fp, fut, ready) in enumerate(self._pending):
+            if self._inflight_bytes + fp <= self.budget:
+                group = [i]
+                gfp = fp
+                if self.max_batch > 1:
+                    key = self._group_key(req)
+                    for j in range(i + 1, len(self._pending)):
+                        if len(group) == self.max_batch:
+                            break
+                        if self._group_key(self._pending[j][0]) != key:
+                            continue
+                        f = self.footprint(req, len(group) + 1)
+                        if self._inflight_bytes + f > self.budget:
+                            break
+                        group.append(j)
+                        gfp = f
+                items = [self._pending[j] for j in group]
+                for j in reversed(group):
+                    del self._pending[j]
+                return items, gfp
             if now - req.arrival > self.max_wait_s:
                 return None  # the oldest waiter blocks later admissions until it fits
         return None
@@ -215,55 +263,84 @@ class SolveServer:
         stream = torch.cuda.Stream(device=self.store.device)
         while True:
             with self._cv:
-                item = None
+                taken = None
                 while not self._stop:
-                    item = self._take()
-                    if item is not None:
+                    taken = self._take()
+                    if taken is not None:
                         break
                     self._cv.wait(timeout=0.05)
-                if item is None:
+                if taken is None:
                     return
-                req, fp, fut = item
+                items, fp = taken
                 self._inflight_bytes += fp
-                self._inflight += 1
+                self._inflight += len(items)
                 self.max_inflight_seen = max(self.max_inflight_seen, self._inflight)
+                self.batches[len(items)] += 1
             try:
                 with torch.cuda.stream(stream):
-                    res = self._solve(req, fp, stream)
-                fut.set_result(res)
-            except Exception as e:  # surfaces on the future
-                fut.set_exception(e)
+                    results = self._solve(items, fp, stream)
+                for (_, _, fut, _), res in zip(items, results):
+                    fut.set_result(res)
+            except Exception as e:  # surfaces on the futures
+                for _, _, fut, _ in items:
+                    if not fut.done():
+                        fut.set_exception(e)
             finally:
                 with self._cv:
                     self._inflight_bytes -= fp
-                    self._inflight -= 1
+                    self._inflight -= len(items)
                     self._cv.notify_all()
 
-    def _solve(self, req, fp, stream):
+    def _solve(self, items, fp, stream):
+        """One lym_cgm call (one request) or one lym_cgm_batched call (a group
+        of requests with one operator and shift list) on the slot's stream."""
         t0 = time.perf_counter()
         lib = _lib.lib()
-        d = self.store.desc(req.field, req.kappa)
-        shifts = [float(s) for s in req.shifts]
+        req0 = items[0][0]
+        d = self.store.desc(req0.field, req0.kappa)
+        shifts = [float(s) for s in req0.shifts]
         n = len(shifts)
+        nrhs = len(items)
         dev = self.store.device
-        src = to_device(req.source, device=dev)
-        if tuple(src.shape) != self.store.spinor_shape(req.field):
-            raise RuntimeError(f"source shape {tuple(src.shape)} does not match field {req.field}")
+        srcs = []
+        for req, _, _, ready in items:
+            if ready is not None:
+                stream.wait_event(ready)
+            src = to_device(req.source, device=dev)
+            if src.is_cuda and isinstance(req.source, torch.Tensor) and req.source.is_cuda:
+                src.record_stream(stream)  # the caller's tensor is in use on this stream
+            if tuple(src.shape) != self.store.spinor_shape(req.field):
+                raise RuntimeError(f"source shape {tuple(src.shape)} does not match field {req.field}")
+            srcs.append(src)
         prec = self._prec_for(n)
-        nbytes = int(lib.lym_solver_workspace_bytes(ctypes.byref(d), prec, n))
-        ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=dev)
-        xs = [torch.empty_like(src) for _ in shifts]
+        xs = [[torch.empty_like(src) for _ in shifts] for src in srcs]
         sh = (ctypes.c_double * n)(*shifts)
-        xp = (ctypes.c_void_p * n)(*[x.data_ptr() for x in xs])
-        st = _lib.LymSolveStats()
-        rc = _lib.check(lib.lym_cgm(ctypes.byref(d), prec, n, sh, _lib.ptr(src), xp, float(req.tolerance),
-                                    self.max_iter, _lib.ptr(ws), ws.numel(), ctypes.byref(st),
-                                    ctypes.c_void_p(stream.cuda_stream)), "serve")
+        xp = (ctypes.c_void_p * (nrhs * n))(*[x.data_ptr() for row in xs for x in row])
+        if nrhs == 1:
+            nbytes = int(lib.lym_solver_workspace_bytes(ctypes.byref(d), prec, n))
+            ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=dev)
+            st = (_lib.LymSolveStats * 1)()
+            rc = _lib.check(lib.lym_cgm(ctypes.byref(d), prec, n, sh, _lib.ptr(srcs[0]), xp, float(req0.tolerance),
+                                        self.max_iter, _lib.ptr(ws), ws.numel(), st,
+                                        ctypes.c_void_p(stream.cuda_stream)), "serve")
+        else:
+            nbytes = int(lib.lym_cgm_batched_workspace_bytes(ctypes.byref(d), prec, n, nrhs))
+            ws = torch.empty(nbytes + 256, dtype=torch.uint8, device=dev)
+            st = (_lib.LymSolveStats * nrhs)()
+            bp = (ctypes.c_void_p * nrhs)(*[s.data_ptr() for s in srcs])
+            rc = _lib.check(lib.lym_cgm_batched(ctypes.byref(d), prec, nrhs, n, sh, bp, xp, float(req0.tolerance),
+                                                self.max_iter, _lib.ptr(ws), ws.numel(), st,
+                                                ctypes.c_void_p(stream.cuda_stream)), "serve batched")
         stream.synchronize()
         done = time.perf_counter()
-        return SolveResult(rid=req.rid, solutions=xs, stats=st.as_dict(n),
-                           status={0: "converged", 1: "max_iterations", 2: "breakdown"}.get(rc, str(rc)),
-                           arrival=req.arrival, start=t0, done=done, footprint=fp)
+        del rc  # worst status of the group; each request gets its own from stats
+        out = []
+        for i, (req, _, _, _) in enumerate(items):
+            stats = st[i].as_dict(n)
+            out.append(SolveResult(rid=req.rid, solutions=xs[i], stats=stats, status=stats["status"],
+                                   arrival=req.arrival, start=t0, done=done, footprint=fp,
+                                   extra={"batch": nrhs}))
+        return out
 
 
 def latency_summary(results):
