@@ -1,4 +1,6 @@
-// Scalar recurrences run by thread 0 of the last block of a reducing kernel.
+// Scalar recurrences of the solvers, run by thread 0 of the one-block
+// finalizer kernel after it has summed the partials of the preceding
+// reducing kernel.
 #pragma once
 #include <math.h>
 
@@ -6,12 +8,38 @@
 
 namespace lym {
 
+// Per-shift inputs of the multishift recurrences.  The finalizer block
+// stages them in LDS, one thread per shift, before thread 0 runs the serial
+// recurrence: thread 0 then reads LDS instead of walking a dependent chain
+// of global loads per shift (stores to the state keep those loads from
+// being overlapped), and it still makes every write to the state itself.
+struct ShiftIn {
+  double zeta[kMaxShifts], zeta_old[kMaxShifts], zeta_new[kMaxShifts], alpha_s[kMaxShifts];
+  double delta[kMaxShifts], tol2_s[kMaxShifts];
+  int active[kMaxShifts];
+};
+
+// Every thread of the block calls this (one barrier); kMaxShifts <= block size.
+__device__ inline void stage_shifts(const SolverState* st, ShiftIn& in) {
+  const int s = (int)threadIdx.x;
+  if (s < st->nshift) {
+    in.zeta[s] = st->zeta[s];
+    in.zeta_old[s] = st->zeta_old[s];
+    in.zeta_new[s] = st->zeta_new[s];
+    in.alpha_s[s] = st->alpha_s[s];
+    in.delta[s] = st->delta[s];
+    in.tol2_s[s] = st->tol2_s[s];
+    in.active[s] = st->active[s];
+  }
+  __syncthreads();
+}
+
 // CG-M after <p, (A+s0) p> is known: step length alpha and the shifted
 // zeta_{k+1}, alpha^s_k (Jegerlehner hep-lat/9612014):
 //   zeta_{k+1} = zeta_k zeta_{k-1} a_{k-1} /
 //                ( a_k b_{k-1} (zeta_{k-1} - zeta_k) + zeta_{k-1} a_{k-1} (1 + d_s a_k) )
 //   alpha^s_k  = a_k zeta_{k+1} / zeta_k
-__device__ inline void cgm_fin_a(SolverState* st, double tt, double pp) {
+__device__ inline void cgm_fin_a(SolverState* st, const ShiftIn& in, double tt, double pp) {
   const double pAp = tt + st->sigma0 * pp;
   if (!(pAp > 0.0) || !isfinite(pAp)) {
     st->status = ST_BREAKDOWN;
@@ -23,9 +51,9 @@ __device__ inline void cgm_fin_a(SolverState* st, double tt, double pp) {
   st->alpha = alpha;
   const double ao = st->alpha_old, bo = st->beta_old;
   for (int s = 0; s < st->nshift; ++s) {
-    if (!st->active[s]) continue;
-    const double z = st->zeta[s], zo = st->zeta_old[s];
-    const double den = alpha * bo * (zo - z) + zo * ao * (1.0 + st->delta[s] * alpha);
+    if (!in.active[s]) continue;
+    const double z = in.zeta[s], zo = in.zeta_old[s];
+    const double den = alpha * bo * (zo - z) + zo * ao * (1.0 + in.delta[s] * alpha);
     const double zn = z * zo * ao / den;
     st->zeta_new[s] = zn;
     st->alpha_s[s] = alpha * zn / z;
@@ -33,35 +61,24 @@ __device__ inline void cgm_fin_a(SolverState* st, double tt, double pp) {
   st->phase = PH_K2;
 }
 
-// CG-M after <r_{k+1}, r_{k+1}> is known: beta, beta^s, roll the recurrence,
-// per-shift convergence (|zeta_s|^2 rr <= tol2_s), stop decision.
-__device__ inline void cgm_fin_b(SolverState* st, double rrn) {
-  const double beta = rrn / st->rr;
-  for (int s = 0; s < st->nshift; ++s) {
-    st->upd[s] = st->active[s];
-    if (!st->active[s]) continue;
-    const double q = st->zeta_new[s] / st->zeta[s];
-    st->beta_s[s] = beta * q * q;
-    st->zeta_old[s] = st->zeta[s];
-    st->zeta[s] = st->zeta_new[s];
-  }
-  st->alpha_old = st->alpha;
-  st->beta_old = beta;
-  st->rr = rrn;
-  st->iter += 1;
+// Convergence of the shifts after <r_{k+1}, r_{k+1}> (the zetas already
+// rolled: zeta_s = in.zeta_new[s]): |zeta_s|^2 rr <= tol2_s, the base last
+// (it carries the common Krylov space: it stops only after all others);
+// status and final_k3 of the solve.
+__device__ inline void cgm_converged(SolverState* st, const ShiftIn& in, double rrn) {
   int nact = 0;
   for (int s = 0; s < st->nshift; ++s) {
-    if (!st->active[s] || s == st->base) continue;
-    if (st->zeta[s] * st->zeta[s] * rrn <= st->tol2_s[s]) {
+    if (!in.active[s] || s == st->base) continue;
+    const double zn = in.zeta_new[s];
+    if (zn * zn * rrn <= in.tol2_s[s]) {
       st->active[s] = 0;
       st->conv_iter[s] = st->iter;
     } else {
       ++nact;
     }
   }
-  // the base carries the common Krylov space: it stops only after all others
-  if (st->active[st->base]) {
-    if (nact == 0 && rrn <= st->tol2_s[st->base]) {
+  if (in.active[st->base]) {
+    if (nact == 0 && rrn <= in.tol2_s[st->base]) {
       st->active[st->base] = 0;
       st->conv_iter[st->base] = st->iter;
     } else {
@@ -79,6 +96,26 @@ __device__ inline void cgm_fin_b(SolverState* st, double rrn) {
     st->status = ST_MAXITER;
     st->final_k3 = 1;
   }
+}
+
+// CG-M after <r_{k+1}, r_{k+1}> is known: beta, beta^s, roll the recurrence,
+// per-shift convergence, stop decision.
+__device__ inline void cgm_fin_b(SolverState* st, const ShiftIn& in, double rrn) {
+  const double beta = rrn / st->rr;
+  for (int s = 0; s < st->nshift; ++s) {
+    st->upd[s] = in.active[s];
+    if (!in.active[s]) continue;
+    const double z = in.zeta[s], zn = in.zeta_new[s];
+    const double q = zn / z;
+    st->beta_s[s] = beta * q * q;
+    st->zeta_old[s] = z;
+    st->zeta[s] = zn;
+  }
+  st->alpha_old = st->alpha;
+  st->beta_old = beta;
+  st->rr = rrn;
+  st->iter += 1;
+  cgm_converged(st, in, rrn);
   st->phase = PH_K3;
 }
 
@@ -86,54 +123,26 @@ __device__ inline void cgm_fin_b(SolverState* st, double rrn) {
 // this iteration are recorded as step blk_j of the current block instead of
 // being applied now; the block is flushed (PH_K3) when full or when the solve
 // stops, otherwise the next iteration starts with p = r + beta p (PH_KP).
-__device__ inline void cgm_fin_b_fast(SolverState* st, double rrn) {
+__device__ inline void cgm_fin_b_fast(SolverState* st, const ShiftIn& in, double rrn) {
   const double beta = rrn / st->rr;
   const int j = st->blk_j;
   for (int s = 0; s < st->nshift; ++s) {
-    if (!st->active[s]) continue;
-    const double q = st->zeta_new[s] / st->zeta[s];
-    const double bs = beta * q * q;
-    st->h_alpha[s][j] = st->alpha_s[s];
-    st->h_zeta[s][j] = st->zeta_new[s];
-    st->h_beta[s][j] = bs;
+    if (!in.active[s]) continue;
+    const double z = in.zeta[s], zn = in.zeta_new[s];
+    const double q = zn / z;
+    st->h_alpha[s][j] = in.alpha_s[s];
+    st->h_zeta[s][j] = zn;
+    st->h_beta[s][j] = beta * q * q;
     st->n_upd[s] = j + 1;
-    st->zeta_old[s] = st->zeta[s];
-    st->zeta[s] = st->zeta_new[s];
+    st->zeta_old[s] = z;
+    st->zeta[s] = zn;
   }
   st->alpha_old = st->alpha;
   st->beta_old = beta;
   st->rr = rrn;
   st->iter += 1;
   st->blk_j = j + 1;
-  int nact = 0;
-  for (int s = 0; s < st->nshift; ++s) {
-    if (!st->active[s] || s == st->base) continue;
-    if (st->zeta[s] * st->zeta[s] * rrn <= st->tol2_s[s]) {
-      st->active[s] = 0;
-      st->conv_iter[s] = st->iter;
-    } else {
-      ++nact;
-    }
-  }
-  if (st->active[st->base]) {
-    if (nact == 0 && rrn <= st->tol2_s[st->base]) {
-      st->active[st->base] = 0;
-      st->conv_iter[st->base] = st->iter;
-    } else {
-      ++nact;
-    }
-  }
-  st->nactive = nact;
-  if (!isfinite(rrn)) {
-    st->status = ST_BREAKDOWN;
-    st->final_k3 = 1;
-  } else if (nact == 0) {
-    st->status = ST_CONVERGED;
-    st->final_k3 = 1;
-  } else if (st->iter >= st->max_iter) {
-    st->status = ST_MAXITER;
-    st->final_k3 = 1;
-  }
+  cgm_converged(st, in, rrn);
   if (rrn <= st->switch_rr) st->want_switch = 1;
   st->phase = (st->final_k3 || st->blk_j >= st->mblk) ? PH_K3 : PH_KP;
 }

@@ -133,6 +133,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       break;
     default: break;
   }
+  __shared__ ShiftIn shifts;
+  if (kind == FK_CGM_A || kind == FK_CG_A || kind == FK_CGM_B || kind == FK_CGM_BF) stage_shifts(st, shifts);
   if (threadIdx.x != 0) return;
   switch (kind) {
     case FK_SUM:
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       }
       break;
     case FK_CGM_A:
-      cgm_fin_a(st, sums[0], sums[1]);
+      cgm_fin_a(st, shifts, sums[0], sums[1]);
       // a breakdown ends the solve here, not at a block flush: count it
       if (ctl && st->phase == PH_DONE) {
         atomicAdd(&ctl->ndone, 1);
@@ -167,9 +169,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
         }
       }
       break;
-    case FK_CGM_B: cgm_fin_b(st, sums[0]); break;
+    case FK_CGM_B: cgm_fin_b(st, shifts, sums[0]); break;
     case FK_CGM_BF:
-      cgm_fin_b_fast(st, sums[0]);
+      cgm_fin_b_fast(st, shifts, sums[0]);
       if (ctl && st->want_switch && !st->switch_counted) {
         st->switch_counted = 1;
         atomicAdd(&ctl->nswitch, 1);
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
     case FK_CG_A:
     case FK_CG_B:
     case FK_CG_RU:
-      if (kind == FK_CG_A) cgm_fin_a(st, sums[0], sums[1]);
+      if (kind == FK_CG_A) cgm_fin_a(st, shifts, sums[0], sums[1]);
       else if (kind == FK_CG_B) cg_fin_b_rel(st, sums[0]);
       else if (kind == FK_CG_RU) cg_fin_ru(st, sums[0], sums[1], sums[2]);
       if (ctl && st->phase == PH_DONE) atomicAdd(&ctl->ndone, 1);
