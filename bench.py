@@ -17,6 +17,10 @@ true residual, the roofline of the solve's dominant kernel (HIP-event
 durations of its launches over a further solve of the same system after the
 timed region, algorithmic bytes), and the CPU baseline (torch-CPU complex128
 restatement of the reference's op sequence, 4 threads and all cores).
+Beside it, the default line carries the 24^3 x 48 measurements: D and the
+force sum with their rooflines, the batched 28-source CG (16-bit storage),
+and the batched multishift CG over 8 sources x 12 shifts, each batch with
+its per-kernel table and the roofline of its dominant and dslash kernels.
 Serving workloads (c4, c4e) report capacity, the sustained rate at the
 offered rate, and latency at 70% of capacity.
 """
@@ -178,7 +182,7 @@ def run_serving(args):
     print(json.dumps(out))
 
 
-def batched_result(workload, precision, steps, warmup, baseline_steps, autotune=True):
+def batched_result(workload, precision, steps, warmup, baseline_steps, autotune=True, timing=True):
     """Batched multi-RHS CG (all sources in the same launches; 16-bit Krylov
     storage with FP32 arithmetic and FP64 reliable updates unless `precision`
     says otherwise) against the plain FP64 solver run source by source, on
@@ -218,6 +222,27 @@ def batched_result(workload, precision, steps, warmup, baseline_steps, autotune=
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
     value = steps * len(srcs) / total
+    # per-kernel durations: one further batch with HIP events around every
+    # launch (direct launches, not part of the timed region)
+    kernels = roofline = dslash_roofline = None
+    if timing:
+        _lib.kernel_timing(True)
+        tk = time.perf_counter()
+        solver.solve_batch(A, srcs, shifts)
+        torch.cuda.synchronize()
+        kt_total = time.perf_counter() - tk
+        _lib.kernel_timing(False)
+        kt = _lib.kernel_timing_read()
+        if kt:
+            kernels, roofline, dslash_roofline = kernel_table(
+                kt, kt_total, W["name"],
+                "HIP events on the solve stream around every launch of the class, over one further batch solve "
+                "after the timed region (direct launches; the timed solves replay HIP graphs)")
+            for tag, r in (("dominant kernel", roofline), ("dslash", dslash_roofline)):
+                if r:
+                    print(f"[bench] batched {workload} {tag} {r['kernel']}: {r['launches']} launches, "
+                          f"{r['avg_us']:.1f} us avg, {r['achieved']:.0f} GB/s ({100 * r['frac']:.1f}%)",
+                          file=sys.stderr, flush=True)
     plain = None
     if baseline_steps > 0:
         ps = dirac_solver.multishift_solver(20000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="double",
@@ -245,13 +270,16 @@ def batched_result(workload, precision, steps, warmup, baseline_steps, autotune=
         "plain_fp64_single_rhs_solves_per_s": plain,
         "speedup_vs_plain_fp64": (value / plain) if plain else None,
         "wilson_tiles": tiles,
+        "roofline": roofline,
+        "dslash_roofline": dslash_roofline,
+        "kernels": kernels,
     }
 
 
 def run_batched(args):
     """The batched workloads as the bench's own JSON line."""
     r = batched_result(args.workload, args.precision, args.steps, args.warmup, args.baseline_steps,
-                       autotune=not args.no_autotune)
+                       autotune=not args.no_autotune, timing=not args.no_kernel_timing)
     out = {
         "metric": r["metric"], "value": r["value"], "unit": r["unit"], "n_gpus": 1, "steps": r["steps"],
         "warmup": r["warmup"], "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -260,6 +288,7 @@ def run_batched(args):
         "config": r["config"], "iterations": r["iterations"],
         "plain_fp64_single_rhs_solves_per_s": r["plain_fp64_single_rhs_solves_per_s"],
         "speedup_vs_plain_fp64": r["speedup_vs_plain_fp64"], "wilson_tiles": r["wilson_tiles"],
+        "roofline": r["roofline"], "dslash_roofline": r["dslash_roofline"], "kernels": r["kernels"],
     }
     print(json.dumps(out))
 
@@ -543,20 +572,50 @@ KERNEL_OF_CLASS = {
     "wilson_cgm_k2f_f16": "k_wilson_tile<..., 2, EpiCgmK2F<2, 12>, ...>",
     "direction_update_f64": "k_fast_kp<0>", "direction_update_f32": "k_fast_kp<1>",
     "direction_update_f16": "k_fast_kp16<12>", "finalize": "k_finalize",
+    "wilson_cg_k1_f16": "k_wilson_tile<..., 2, EpiCgmK1<2, 12>, ...>",
+    "wilson_cg_k2x_f16": "k_wilson_tile<..., 2, EpiCgK2X<2, 12>, ...>",
+    "wilson_cg_k1_f32": "k_wilson_tile<..., 1, EpiCgmK1<1, 12>, ...>",
+    "wilson_cg_k2x_f32": "k_wilson_tile<..., 1, EpiCgK2X<1, 12>, ...>",
+    "cg_direction_update_f16": "k_rel_kp<2, 12>", "cg_direction_update_f32": "k_rel_kp<1, 12>",
+    "reliable_update": "k_ru_add + k_wilson_tile<..., 0, EpiRu1/EpiRu2H> + k_finalize (gated per RHS)",
 }
+
+
+def kernel_table(kt, kt_total, workload, timing_note):
+    """Per-class durations of a kernel-timed solve (lym_kernel_timing) as the
+    bench reports them, the roofline of the dominant class with algorithmic
+    bytes, and the roofline of the dominant Wilson (dslash) class."""
+    kernels = {k: dict(v, avg_us=v["us"] / max(1, v["launches"]),
+                       gbs=(v["bytes"] / (v["us"] * 1e-6) / 1e9) if v["us"] > 0 and v["bytes"] > 0 else None,
+                       share=v["us"] / (kt_total * 1e6))
+               for k, v in kt.items()}
+
+    def roof(cls):
+        d = kt[cls]
+        gbs = d["bytes"] / (d["us"] * 1e-6) / 1e9
+        return {"kernel": f"{cls} ({KERNEL_OF_CLASS.get(cls, cls)})", "bound": "hbm", "achieved": gbs,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                "traffic": measured_class_traffic(workload, cls),
+                "bytes_per_launch": d["bytes"] / d["launches"], "avg_us": d["us"] / d["launches"],
+                "launches": d["launches"], "share_of_solve": d["us"] / (kt_total * 1e6), "timing": timing_note}
+    with_bytes = [k for k in kt if kt[k]["bytes"] > 0]
+    dom = max(with_bytes, key=lambda k: kt[k]["us"]) if with_bytes else None
+    wil = [k for k in with_bytes if k.startswith("wilson")]
+    dwil = max(wil, key=lambda k: kt[k]["us"]) if wil else None
+    return kernels, (roof(dom) if dom else None), (roof(dwil) if dwil else None)
 
 
 def measured_class_traffic(workload, cls):
     """HBM bytes per launch of kernel class `cls` in `workload` from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, or None."""
-    path = os.path.join(ROOT, "profiles", "multishift_c5_traffic.md")
-    if not os.path.exists(path):
-        return None
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries (profiles/*traffic*.md),
+    or None."""
+    import glob
     import re
-    for line in open(path):
-        m = re.search(r"traffic_bytes_per_launch (\S+) (\S+) = (\d+)", line)
-        if m and m.group(1) == workload and m.group(2) == cls:
-            return int(m.group(3))
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
+        for line in open(path):
+            m = re.search(r"traffic_bytes_per_launch (\S+) (\S+) = (\d+)", line)
+            if m and m.group(1) == workload and m.group(2) == cls:
+                return int(m.group(3))
     return None
 
 
@@ -574,7 +633,8 @@ def main():
     ap.add_argument("--dslash-reps", type=int, default=50)
     ap.add_argument("--no-dslash-c3", action="store_true", help="skip the 24^3x48 dslash measurement")
     ap.add_argument("--no-batched", action="store_true",
-                    help="skip the batched 28-source CG at 24^3x48 reported beside the default workload")
+                    help="skip the batched 28-source CG and the batched 8-source 12-shift multishift CG at 24^3x48 "
+                         "reported beside the default workload")
     ap.add_argument("--dslash-geometry", default="24x24x24x48", help="lattice of --workload dslash")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in Wilson tile rule")
     ap.add_argument("--no-kernel-timing", action="store_true",
@@ -685,23 +745,14 @@ def main():
     roofline = None
     kernels = None
     if kt:
-        kernels = {k: dict(v, avg_us=v["us"] / max(1, v["launches"]),
-                           gbs=(v["bytes"] / (v["us"] * 1e-6) / 1e9) if v["us"] > 0 and v["bytes"] > 0 else None,
-                           share=v["us"] / (kt_total * 1e6))
-                   for k, v in kt.items()}
-        dom = max((k for k in kt if kt[k]["bytes"] > 0), key=lambda k: kt[k]["us"])
-        d = kt[dom]
-        gbs = d["bytes"] / (d["us"] * 1e-6) / 1e9
-        roofline = {"kernel": f"{dom} ({KERNEL_OF_CLASS.get(dom, dom)})", "bound": "hbm", "achieved": gbs,
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                    "traffic": measured_class_traffic(W["name"], dom),
-                    "bytes_per_launch": d["bytes"] / d["launches"], "avg_us": d["us"] / d["launches"],
-                    "launches": d["launches"], "share_of_solve": d["us"] / (kt_total * 1e6),
-                    "timing": f"HIP events on the solve stream around every launch of the class, over "
-                              f"{args.timing_steps} further solve(s) of the benchmarked system after the timed region "
-                              f"(direct launches; the timed solves replay HIP graphs)"}
-        print(f"[bench] dominant kernel {dom}: {d['launches']} launches, {d['us'] / d['launches']:.1f} us avg, "
-              f"{gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of {HBM_PEAK_GBS:.0f})", file=sys.stderr, flush=True)
+        kernels, roofline, _ = kernel_table(
+            kt, kt_total, W["name"],
+            f"HIP events on the solve stream around every launch of the class, over {args.timing_steps} further "
+            f"solve(s) of the benchmarked system after the timed region (direct launches; the timed solves replay "
+            f"HIP graphs)")
+        print(f"[bench] dominant kernel {roofline['kernel']}: {roofline['launches']} launches, "
+              f"{roofline['avg_us']:.1f} us avg, {roofline['achieved']:.0f} GB/s "
+              f"({100 * roofline['frac']:.1f}% of {HBM_PEAK_GBS:.0f})", file=sys.stderr, flush=True)
 
     if args.calib_copy:
         calib_copy()
@@ -720,7 +771,13 @@ def main():
     # ---- the batched 28-source CG at 24^3x48 (16-bit storage) beside it ---
     res_b = None
     if not args.no_batched:
-        res_b = batched_result("c3b", None, 1, 1, 2, autotune=args.no_dslash_c3 and not args.no_autotune)
+        res_b = batched_result("c3b", None, 1, 1, 2, autotune=args.no_dslash_c3 and not args.no_autotune,
+                               timing=not args.no_kernel_timing)
+    # ---- the batched multishift CG at 24^3x48 (8 sources x 12 shifts, the
+    # mixed-precision batched multishift solve of the dslash target) ---------
+    res_m = None
+    if not args.no_batched:
+        res_m = batched_result("c3m", None, 1, 1, 2, autotune=False, timing=not args.no_kernel_timing)
 
     # ---- CPU baseline -----------------------------------------------------
     cpu = None
@@ -757,6 +814,7 @@ def main():
         "dslash": res,
         "dslash_24c3x48": res_c3,
         "batched_24c3x48": res_b,
+        "batched_multishift_24c3x48": res_m,
         "wilson_tiles": tiles,
         "cpu_baseline": cpu,
     }

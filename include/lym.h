@@ -277,16 +277,18 @@ int lym_bicgstab(const lym_wilson* op, int square, double shift, const void* b, 
  * residual of every shift into stats->true_rel_resid.
  *   prec = LYM_PREC_F64: FP64 throughout.
  *   prec = LYM_PREC_F32, several shifts: FP64 Krylov vectors until the base
- *          residual is below 1e-4 |b|, complex64 after; shift solutions and
+ *          residual is below 3e-4 |b|, complex64 after; shift solutions and
  *          directions accumulate in FP64 (deferred block updates); FP64
  *          defect correction of any shift whose true residual misses tol.
+ *   prec = LYM_PREC_F16, several shifts, fundamental representation: as
+ *          LYM_PREC_F32, then 16-bit storage of the Krylov vectors (int16
+ *          fixed point scaled by a per-site float norm, 4 bytes per complex
+ *          number; FP32 arithmetic) below 1e-6 |b|.
  *   prec = LYM_PREC_F32, one shift: complex64 CG with FP64 reliable updates
  *          (true residual recomputed in FP64 whenever the iterated one has
- *          dropped 10x, and at the end).
+ *          dropped 100x, and at the end).
  *   prec = LYM_PREC_F16, one shift, fundamental representation: the same
- *          with 16-bit storage of the Krylov vectors (int16 fixed point
- *          scaled by a per-site float norm, 4 bytes per complex number);
- *          arithmetic in FP32.
+ *          with 16-bit storage of r, p and Q p; arithmetic in FP32.
  * b and xs are complex128.  Synchronous.  New entry point (no counterpart in
  * the reference, whose multishift solver is the MR above); it serves
  * multishift_solver.solve / RationalApproximation.apply (dirac.py:52-59). */

@@ -135,6 +135,13 @@ __device__ __forceinline__ T buf_ld(const T* base, uint32_t bytes, int64_t plane
   return out;
 }
 
+// Cache policy of the field stores: nontemporal (the NT bit of the buffer
+// instruction's cache-policy operand).  Outputs are written once per launch;
+// measured on MI355X, nontemporal stores shorten the Wilson kernels by 3-6%
+// (a D launch at 24^3x48 and 32^3x64, complex64), while nontemporal loads
+// lengthen them (neighbours re-read the lines through L2).
+constexpr int kStorePolicy = 2;
+
 template <typename T>
 __device__ __forceinline__ void buf_st(T* base, uint32_t bytes, int64_t plane, int64_t V, int64_t s, const T& val) {
   const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
@@ -149,16 +156,16 @@ __device__ __forceinline__ void buf_st(T* base, uint32_t bytes, int64_t plane, i
     // per SIMD); with soffset = 0 the compiler's hazard guard applies.
     __attribute__((ext_vector_type(4))) unsigned int v;
     __builtin_memcpy(&v, &val, 16);
-    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff + soff, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, voff + soff, 0, kStorePolicy);
   } else if constexpr (sizeof(T) == 8) {
     __attribute__((ext_vector_type(2))) unsigned int v;
     __builtin_memcpy(&v, &val, 8);
-    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, kStorePolicy);
   } else {
     static_assert(sizeof(T) == 4, "buffer element size");
     unsigned int v;
     __builtin_memcpy(&v, &val, 4);
-    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, kStorePolicy);
   }
 }
 

@@ -872,12 +872,20 @@ static Field take_field(Carver& c, const OpDesc& d, int prec) {
 // (switch16) take over, and the iterated tolerance as a fraction of the
 // target (the FP64 true residual decides at the end).  A rounding error of
 // relative size e made while the base residual is r shows up in the shifted
-// true residuals at about e r, summed over the iterations of the phase:
-// complex64 from 1e-4 and 16-bit from 1e-7 keep that sum below the target.
+// true residuals at about e r, summed over the iterations of the phase; a
+// shift that still misses the target gets FP64 defect correction.  Measured
+// on MI355X (solves/s; every shift to 1e-10 after correction):
+//   switch32, switch16   32^3x64, 24 shifts   24^3x48, 8 sources x 12 shifts
+//   3e-5, 1e-7           1.099                5.22
+//   1e-4, 1e-7           1.125                5.45
+//   1e-4, 1e-6           1.123                5.53
+//   3e-4, 1e-7           1.059                5.57
+//   3e-4, 1e-6           1.173                5.72
+//   1e-3, 1e-6           0.866 (8 corrections, 519 it)   5.77
 // LYM_CGM_TUNE="switch32,switch16,iter_factor" overrides them (tuning runs).
 struct CgmTuning {
-  double switch32 = 1e-4;
-  double switch16 = 1e-7;
+  double switch32 = 3e-4;
+  double switch16 = 1e-6;
   double iter_factor = 0.5;
   int mblk_lo = kK3BlockLo;  // block length of the complex64 / 16-bit phases
   int mblk_64 = kK3Block64;  // block length of the FP64 phase
@@ -1383,14 +1391,18 @@ int kernel_timing_read(lym_kernel_time* out, int max, int* count) {
   return LYM_OK;
 }
 
-// algorithmic bytes per site of one Wilson launch: links (D directions x
-// rows x nc complex) + `nvec` spinors (complex numbers of 2 * ws bytes, plus
-// a 4-byte norm per site for 16-bit storage)
-static double wilson_bytes_site(const OpDesc& d, int prec, int nvec) {
+// kernel-class suffix of a storage precision
+static const char* pname(int p) { return p == P64 ? "f64" : p == P32 ? "f32" : "f16"; }
+
+// algorithmic bytes of one Wilson launch over nrhs right-hand sides: the
+// links once (D directions x rows x nc complex; one read serves every RHS of
+// the launch) + `nvec` spinors per RHS (complex numbers of 2 * ws bytes, plus
+// a 4-byte norm per site for 16-bit storage), per site
+static double wilson_bytes(const OpDesc& d, int prec, int nvec, int nrhs) {
   const double wl = prec == P64 ? 8.0 : 4.0;
   const double rows = (!d.adj && d.link_su) ? d.nc - 1 : d.nc;
   const double links = (double)d.ndim * rows * d.nc * 2.0 * wl * (d.adj ? 2.0 : 1.0);
-  return links + nvec * vec_bytes_site(d, prec);
+  return (double)d.lat.V * (links + (double)nrhs * nvec * vec_bytes_site(d, prec));
 }
 
 // Fast multishift CG over a batch of nrhs right-hand sides (one shift list).
@@ -1470,7 +1482,6 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   int prec = P64;
   // chunks issued so far (precision and length), for the kernel timing
   std::vector<std::pair<int, int>> chunk_log;
-  auto pname = [](int p) { return p == P64 ? "f64" : p == P32 ? "f32" : "f16"; };
   auto chunk = [&](hipStream_t st) -> int {
     const Field ring = prec == P64 ? P.ring64 : prec == P32 ? P.ring32 : P.ring16;
     const Field pl = prec == P64 ? P.plive64 : prec == P32 ? P.plive32 : P.plive16;
@@ -1506,7 +1517,7 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
       a1.out = t;
       a1.st = P.st;
       a1.rhs_stride = n;
-      e = kt_launch(st, ("wilson_cgm_k1_" + sfx).c_str(), wilson_bytes_site(d, prec, 2) * sites, -1,
+      e = kt_launch(st, ("wilson_cgm_k1_" + sfx).c_str(), wilson_bytes(d, prec, 2, nrhs), -1,
                     [&]() { return wilson_launch(EPI_CGM_K1, d, prec, true, pl, a1, redctl(P), st, nrhs); });
       if (!e)
         e = kt_launch(st, "finalize", 0.0, -1,
@@ -1520,7 +1531,7 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
       a2.nslot = nslot;
       a2.rhs_stride = n;
       a2.ring_rhs_stride = ring_stride;
-      e = kt_launch(st, ("wilson_cgm_k2f_" + sfx).c_str(), wilson_bytes_site(d, prec, 4) * sites, -1,
+      e = kt_launch(st, ("wilson_cgm_k2f_" + sfx).c_str(), wilson_bytes(d, prec, 4, nrhs), -1,
                     [&]() { return wilson_launch(EPI_CGM_K2F, d, prec, true, t, a2, redctl(P), st, nrhs); });
       if (!e)
         e = kt_launch(st, "finalize", 0.0, -1,
@@ -1792,53 +1803,70 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   // the update accumulation is mostly gated off: a small grid-stride grid
   const int gsr = std::min(gs, 1024);
   static_assert(kRelChunk % 2 == 0, "the last iteration of a chunk must carry the update kernels");
+  // kernel timing: class suffix and the direction update's bytes (r and p
+  // read, p written at the storage precision; x_lo read and written in
+  // complex64)
+  const std::string sfx = pname(pl);
+  const double kp_bytes = (double)V * nrhs * (3.0 * vec_bytes_site(d, pl) + 2.0 * vec_bytes_site(d, P32));
   auto chunk = [&](hipStream_t st) -> int {
     for (int j = 0; j < kRelChunk; ++j) {
       // x_lo += alpha p (last step), p = r + beta p
-      int e = dispatch([&](auto pv, auto nc) {
-        hipLaunchKernelGGL((k_rel_kp<decltype(pv)::value, decltype(nc)::value>), dim3(gs, nrhs), dim3(kBlock), 0,
-                           st, P.st, V, P.rl_r, P.rl_p, P.rl_x);
+      int e = kt_launch(st, ("cg_direction_update_" + sfx).c_str(), kp_bytes, -1, [&]() {
+        return dispatch([&](auto pv, auto nc) {
+          hipLaunchKernelGGL((k_rel_kp<decltype(pv)::value, decltype(nc)::value>), dim3(gs, nrhs), dim3(kBlock), 0,
+                             st, P.st, V, P.rl_r, P.rl_p, P.rl_x);
+        });
       });
       if (e) return e;
       EpiArgs a1;
       a1.out = P.rl_t;
       a1.st = P.st;
       a1.rhs_stride = n;
-      e = wilson_launch(EPI_CGM_K1, db, pl, true, P.rl_p, a1, redctl(P), st, nrhs);
-      if (!e) e = launch_fin(FK_CG_A, P.st, P.partials, gwl, 2, nullptr, st, nrhs, P.ctl);
+      e = kt_launch(st, ("wilson_cg_k1_" + sfx).c_str(), wilson_bytes(d, pl, 2, nrhs), -1,
+                    [&]() { return wilson_launch(EPI_CGM_K1, db, pl, true, P.rl_p, a1, redctl(P), st, nrhs); });
+      if (!e)
+        e = kt_launch(st, "finalize", 0.0, -1,
+                      [&]() { return launch_fin(FK_CG_A, P.st, P.partials, gwl, 2, nullptr, st, nrhs, P.ctl); });
       if (e) return e;
       EpiArgs a2;
       a2.aux0 = P.rl_p;
       a2.aux1 = P.rl_r;
       a2.st = P.st;
       a2.rhs_stride = n;
-      e = wilson_launch(EPI_CG_K2X, db, pl, true, P.rl_t, a2, redctl(P), st, nrhs);
-      if (!e) e = launch_fin(FK_CG_B, P.st, P.partials, gwl, 1, nullptr, st, nrhs, P.ctl);
+      e = kt_launch(st, ("wilson_cg_k2x_" + sfx).c_str(), wilson_bytes(d, pl, sigma != 0.0 ? 4 : 3, nrhs), -1,
+                    [&]() { return wilson_launch(EPI_CG_K2X, db, pl, true, P.rl_t, a2, redctl(P), st, nrhs); });
+      if (!e)
+        e = kt_launch(st, "finalize", 0.0, -1,
+                      [&]() { return launch_fin(FK_CG_B, P.st, P.partials, gwl, 1, nullptr, st, nrhs, P.ctl); });
       if (e) return e;
       // reliable update after every second iteration (an RHS that asks for
       // one after an even iteration skips the next): every kernel below
       // returns at once for RHS not in PH_RU, so their cost is the launch
       if (j % 2 == 0) continue;
-      e = dispatch([&](auto pv, auto nc) {
-        hipLaunchKernelGGL((k_ru_add<decltype(pv)::value, decltype(nc)::value>), dim3(gsr, nrhs), dim3(kBlock), 0,
-                           st, P.st, V, P.rl_x, P.rl_p, (double2*)P.rl_x64.data);
+      // gated: only the RHS in PH_RU do work, so no bytes are attributed
+      e = kt_launch(st, "reliable_update", 0.0, -1, [&]() {
+        int e2 = dispatch([&](auto pv, auto nc) {
+          hipLaunchKernelGGL((k_ru_add<decltype(pv)::value, decltype(nc)::value>), dim3(gsr, nrhs), dim3(kBlock), 0,
+                             st, P.st, V, P.rl_x, P.rl_p, (double2*)P.rl_x64.data);
+        });
+        if (e2) return e2;
+        EpiArgs a3;
+        a3.out = P.rl_t64;
+        a3.st = P.st;
+        a3.rhs_stride = n;
+        e2 = wilson_launch(EPI_RU1, db, P64, true, P.rl_x64, a3, RedCtl{nullptr, nullptr}, st, nrhs);
+        if (e2) return e2;
+        EpiArgs a4;
+        a4.out = P.rl_r;
+        a4.aux0 = P.rl_x64;
+        a4.aux1 = P.rl_p;
+        a4.bs = P.bs_dev;
+        a4.st = P.st;
+        a4.rhs_stride = n;
+        e2 = wilson_launch(pl == P16 ? EPI_RU2H : EPI_RU2, db, P64, true, P.rl_t64, a4, redctl(P), st, nrhs);
+        if (!e2) e2 = launch_fin(FK_CG_RU, P.st, P.partials, gw64, 3, nullptr, st, nrhs, P.ctl);
+        return e2;
       });
-      if (e) return e;
-      EpiArgs a3;
-      a3.out = P.rl_t64;
-      a3.st = P.st;
-      a3.rhs_stride = n;
-      e = wilson_launch(EPI_RU1, db, P64, true, P.rl_x64, a3, RedCtl{nullptr, nullptr}, st, nrhs);
-      if (e) return e;
-      EpiArgs a4;
-      a4.out = P.rl_r;
-      a4.aux0 = P.rl_x64;
-      a4.aux1 = P.rl_p;
-      a4.bs = P.bs_dev;
-      a4.st = P.st;
-      a4.rhs_stride = n;
-      e = wilson_launch(pl == P16 ? EPI_RU2H : EPI_RU2, db, P64, true, P.rl_t64, a4, redctl(P), st, nrhs);
-      if (!e) e = launch_fin(FK_CG_RU, P.st, P.partials, gw64, 3, nullptr, st, nrhs, P.ctl);
       if (e) return e;
     }
     return LYM_OK;
@@ -1881,6 +1909,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   out_states.resize(nrhs);
   LYM_HIP(hipMemcpyAsync(out_states.data(), P.st, sizeof(SolverState) * nrhs, hipMemcpyDeviceToHost, st));
   LYM_HIP(hipStreamSynchronize(st));
+  if (g_kt.on && (rc = kt_collect([](int) { return 0.0; }))) return rc;
   return LYM_OK;
 }
 
