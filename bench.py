@@ -280,6 +280,8 @@ def run_batched(args):
     """The batched workloads as the bench's own JSON line."""
     r = batched_result(args.workload, args.precision, args.steps, args.warmup, args.baseline_steps,
                        autotune=not args.no_autotune, timing=not args.no_kernel_timing)
+    if args.calib_copy:
+        calib_copy()
     out = {
         "metric": r["metric"], "value": r["value"], "unit": r["unit"], "n_gpus": 1, "steps": r["steps"],
         "warmup": r["warmup"], "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -449,33 +451,15 @@ def dslash_c3(reps, geometry=None, autotune=True):
 
 def measured_traffic(geometry, prec):
     """HBM bytes per launch of D at `geometry` / `prec` from the committed
-    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE passes, calibrated on a copy
-    kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "wilson_tile_traffic.md")
-    if not os.path.exists(path):
-        return None
-    key = "x".join(str(L) for L in geometry)
-    import re
-    for line in open(path):
-        m = re.search(r"traffic_bytes_per_launch (\S+) (f32|f64) = (\d+)", line)
-        if m and m.group(1) == key and m.group(2) == prec:
-            return int(m.group(3))
-    return None
+    rocprofv3 summaries (FETCH_SIZE and WRITE_SIZE passes, calibrated on a
+    copy kernel of known size), or None."""
+    return measured_class_traffic("x".join(str(L) for L in geometry), prec)
 
 
 def measured_force_traffic(geometry):
     """HBM bytes per force launch at `geometry` from the committed rocprofv3
-    summary (calibrated FETCH_SIZE + WRITE_SIZE), or None."""
-    path = os.path.join(ROOT, "profiles", "force_traffic.md")
-    if not os.path.exists(path):
-        return None
-    key = "x".join(str(L) for L in geometry)
-    import re
-    for line in open(path):
-        m = re.search(r"traffic_bytes_per_launch (\S+) force = (\d+)", line)
-        if m and m.group(1) == key:
-            return int(m.group(2))
-    return None
+    summaries (calibrated FETCH_SIZE + WRITE_SIZE), or None."""
+    return measured_class_traffic("x".join(str(L) for L in geometry), "force")
 
 
 def calib_copy():
