@@ -364,6 +364,7 @@ def time_dslash(op, src, reps):
     desc = op.ensure_f32()
     nc, V = op.number_of_colors, op.volume
     rows = nc - 1 if desc.link_format == _lib.LINKS_SU else nc
+    adjoint = op.representation == "adjoint"
     res = {}
     for prec, name, w in ((_lib.PREC_F64, "f64", 8), (_lib.PREC_F32, "f32", 4)):
         dt = torch.complex128 if prec == _lib.PREC_F64 else torch.complex64
@@ -374,7 +375,8 @@ def time_dslash(op, src, reps):
             _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), prec, 0, _lib.ptr(x), _lib.ptr(y),
                                                    _lib.stream_handle()))
         us = kernel_time_us(launch, reps)
-        bps = bytes_per_site_dslash(nc, 4, w, w, rows)
+        # adjoint spinors are Ns x Nc x Nc matrices; their links are read in full
+        bps = (4 * nc * nc * 2 * w + 2 * 4 * nc * nc * 2 * w) if adjoint else bytes_per_site_dslash(nc, 4, w, w, rows)
         gbs = bps * V / (us * 1e-6) / 1e9
         res[name] = dict(us=us, bytes_per_site=bps, gbs=gbs, frac=gbs / HBM_PEAK_GBS)
     return res
@@ -445,6 +447,18 @@ def dslash_c3(reps, geometry=None, autotune=True):
     for name in ("f64", "f32"):
         print(f"[bench] wilson {g} {name}: {res[name]['us']:.2f} us/launch, {res[name]['gbs']:.0f} GB/s "
               f"({100 * res[name]['frac']:.1f}% of {HBM_PEAK_GBS:.0f})", file=sys.stderr, flush=True)
+    # the adjoint representation (the reference driver's default, matrix
+    # spinors U X U^dag) on the same links
+    src_a = torch.from_numpy(fields.gaussian_spinor(g, 4, W["nc"], seed=W["seed"] + 2, representation="adjoint")).cuda()
+    op_a = dirac.DiracWilsonOperator(lattice.Configuration(U, g, W["nc"], 4), W["kappa"], True,
+                                     representation="adjoint")
+    tiles_a = op_a.autotune() if autotune else {n: op_a.tile_info(n) for n in ("f64", "f32")}
+    res["adjoint"] = time_dslash(op_a, src_a, reps)
+    res["adjoint"]["tiles"] = tiles_a
+    for name in ("f64", "f32"):
+        r = res["adjoint"][name]
+        print(f"[bench] wilson adjoint {g} {name}: {r['us']:.2f} us/launch, {r['gbs']:.0f} GB/s "
+              f"({100 * r['frac']:.1f}% of {HBM_PEAK_GBS:.0f})", file=sys.stderr, flush=True)
     print(f"[bench] dslash set-up + timing {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     return res
 

@@ -68,18 +68,20 @@ def test_tiled_schedule_parity(geometry, nc, rep, bc):
     assert rel(op.apply(psi), want) < TOL64
 
 
-def _tile_shapes(geometry, es, ncomp):
+def _tile_shapes(geometry, es, ncomp, max_sites=None):
     """Every LDS tile (a0, a1, k) the library admits on `geometry`: a0 divides
     L0/8, a1 | L1, k | L2, a0*a1*k*L3 a multiple of 64 and <= 1024 sites
-    (640 for complex128), tile spinor <= 150 KiB."""
+    (640 for complex128, 256 for adjoint spinors), tile spinor <= 150 KiB."""
     L0, L1, L2, T = geometry
     slab = L0 // 8
+    if max_sites is None:
+        max_sites = 640 if es == 16 else 1024
     out = []
     for a0 in range(1, slab + 1):
         for a1 in range(1, L1 + 1):
             for k in range(1, L2 + 1):
                 ns = a0 * a1 * k * T
-                if slab % a0 or L1 % a1 or L2 % k or ns % 64 or ns > (640 if es == 16 else 1024) or ns * ncomp * es > 150 * 1024:
+                if slab % a0 or L1 % a1 or L2 % k or ns % 64 or ns > max_sites or ns * ncomp * es > 150 * 1024:
                     continue
                 out.append((a0, a1, k))
     return out
@@ -125,6 +127,50 @@ def test_lds_tile_shapes(prec, geometry, monkeypatch):
             d = (got - plain).abs().max().item() / plain.abs().max().item()
             assert d < (1e-15 if prec == "f64" else 1e-6), (a0, a1, k, hif, d)
         monkeypatch.delenv("LYM_HIF")
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("nc", [2, 3])
+@pytest.mark.parametrize("herm,bc", [(True, -1.0), (False, None)])
+def test_lds_tile_shapes_adjoint(prec, nc, herm, bc, monkeypatch):
+    """Adjoint representation (matrix spinors U X U^dag) on the LDS-tiled
+    kernel: every admissible tile shape (at most 256 sites, one hop in
+    flight) gives the oracle's D and the untiled kernel's result to rounding."""
+    import ctypes
+    from tfleonardym_amd import _lib
+    geometry = (16, 4, 4, 16)
+    print(f"case adjoint lds tiles prec={prec} nc={nc} herm={herm} bc={bc}", flush=True)
+    U = fields.gauge_field(geometry, nc, seed=51, kind="haar")
+    psi = fields.gaussian_spinor(geometry, 4, nc, seed=52, representation="adjoint")
+    op = make_op(U, geometry, 0.131, herm, "adjoint", bc)
+    desc = op.ensure_f32()
+    want = ow.WilsonOracle(U, geometry, 0.131, herm, "adjoint", bc_factor=bc).apply(psi)
+    pc = _lib.PREC_F64 if prec == "f64" else _lib.PREC_F32
+    dt = torch.complex128 if prec == "f64" else torch.complex64
+    x = torch.from_numpy(psi).to("cuda").to(dt)
+
+    def run():
+        out = torch.zeros_like(x)
+        _lib.check(_lib.lib().lym_wilson_apply(ctypes.byref(desc), pc, 0, _lib.ptr(x), _lib.ptr(out),
+                                               _lib.stream_handle()))
+        return out
+
+    monkeypatch.setenv("LYM_NO_TILE", "1")
+    plain = run()
+    monkeypatch.setenv("LYM_NO_TILE", "0")
+    info = _lib.LymTileChoice()
+    _lib.check(_lib.lib().lym_tile_info(ctypes.byref(desc), pc, ctypes.byref(info)))
+    assert info.a0 > 0 and info.nsite <= 256, info.as_dict()  # the default runs a tile
+    es = 16 if prec == "f64" else 8
+    shapes = _tile_shapes(geometry, es, 4 * nc * nc, max_sites=256)
+    assert shapes
+    tol = TOL64 if prec == "f64" else TOL32
+    for a0, a1, k in shapes:
+        monkeypatch.setenv("LYM_TILE", f"{a0},{a1},{k}")
+        got = run()
+        assert rel(got.to(torch.complex128), want) < tol, (a0, a1, k)
+        d = (got - plain).abs().max().item() / plain.abs().max().item()
+        assert d < (1e-15 if prec == "f64" else 1e-6), (a0, a1, k, d)
 
 
 def test_autotune_records_and_keeps_parity():

@@ -70,13 +70,14 @@ static size_t fbytes(const OpDesc& d, int prec) {
   return (size_t)field_elems(d) * (prec == LYM_PREC_F64 ? 16 : prec == LYM_PREC_F32 ? 8 : 4);
 }
 
-// Every LDS tile a 4D fundamental lattice admits (L0 divisible by the XCD
-// count): a0 | L0/8, a1 | L1, k | L2, a0*a1*k*L3 a multiple of 64 and at
-// most max_threads sites, lds_per_site * sites within max_lds bytes.
+// Every LDS tile a 4D lattice admits (L0 divisible by the XCD count):
+// a0 | L0/8, a1 | L1, k | L2, a0*a1*k*L3 a multiple of 64 and at most
+// max_threads sites, lds_per_site * sites within max_lds bytes.
 static std::vector<TileMap> admissible_tiles(const lym_wilson* w, int adj, int64_t lds_per_site, int max_threads,
                                              int64_t max_lds) {
+  (void)adj;
   std::vector<TileMap> out;
-  if (w->ndim != 4 || adj || w->dims[0] % kNumXcd) return out;
+  if (w->ndim != 4 || w->dims[0] % kNumXcd) return out;
   const int L0 = w->dims[0], L1 = w->dims[1], L2 = w->dims[2], T = w->dims[3];
   const int64_t V = (int64_t)L0 * L1 * L2 * T;
   const int slab = L0 / kNumXcd;
@@ -210,7 +211,7 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   if (const char* wt = std::getenv("LYM_TILE")) {
     if (std::sscanf(wt, "%d,%d,%d", &want[0], &want[1], &want[2]) != 3) want[0] = want[1] = want[2] = 0;
   }
-  // LDS tile per precision (fundamental 4D, L0 a multiple of the XCD count):
+  // LDS tile per precision (4D, L0 a multiple of the XCD count):
   // minimise global spinor reads per site, 1 + sum_{mu<3} 2/a_mu (0 for a
   // full extent), within 64 KiB of LDS and 640 threads (a multiple of 64).
   // Blocks of 256..512 sites are preferred when the lattice admits them
@@ -241,7 +242,11 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   const int want_hif = hf ? std::atoi(hf) : 0;
   for (int pi = 0; pi < 2; ++pi) {
     const int64_t es = pi == 0 ? 16 : 8;
-    TileMap best = choose_tile(o.ncomp * es, pi == 0 ? kTileMaxThreads64 : kTileMaxThreads, 256, 512, kTileMaxLds);
+    // adjoint tiles: at most kTileAdjThreads sites (register budget), the
+    // largest admissible preferred
+    TileMap best = o.adj ? choose_tile(o.ncomp * es, kTileAdjThreads, 128, kTileAdjThreads, kTileMaxLds)
+                         : choose_tile(o.ncomp * es, pi == 0 ? kTileMaxThreads64 : kTileMaxThreads, 256, 512,
+                                       kTileMaxLds);
     o.tile_tuned[pi] = 0;
     if (!want[0] && !want_hif && o.use_tile) {
       // a shape recorded by lym_tune_wilson for this lattice wins
@@ -267,7 +272,7 @@ int make_desc(const lym_wilson* w, OpDesc* d) {
   // (LYM_MR=1 in the environment): kMrRhs spinor tiles leave LDS room for
   // only a few waves per CU, and measured on MI355X that costs more than the
   // saved link traffic (batched CG, 16^3x32 and 24^3x48: 8% and 17% slower).
-  o.tile_mr = choose_tile(o.ncomp * 8 * kMrRhs, kTileMaxThreadsMr, 128, 256, kTileMaxLdsMr);
+  o.tile_mr = o.adj ? TileMap{} : choose_tile(o.ncomp * 8 * kMrRhs, kTileMaxThreadsMr, 128, 256, kTileMaxLdsMr);
   const char* mr = std::getenv("LYM_MR");
   o.use_mr = (mr && mr[0] == '1') ? 1 : 0;
   *d = o;
@@ -443,8 +448,9 @@ int lym_tune_wilson(const lym_wilson* op, int prec, int reps, const void* in, vo
   // candidates: the untiled kernel, then every tile x hops in flight
   std::vector<TileMap> cands{TileMap{}};
   for (const TileMap& t : admissible_tiles(op, d.adj, (int64_t)d.ncomp * (pi == 0 ? 16 : 8),
-                                           pi == 0 ? kTileMaxThreads64 : kTileMaxThreads, kTileMaxLds)) {
+                                           tile_max_threads(d.adj, pi == 0 ? P64 : P32), kTileMaxLds)) {
     for (int hif : {1, 2, 4}) {
+      if (hif > 1 && d.adj) continue;  // adjoint tiles run one hop in flight
       if (hif == 4 && prec != LYM_PREC_F32) continue;
       // the launcher runs larger tiles with one hop in flight
       if (hif > 1 && t.nsite > tile_hif_threads(prec == LYM_PREC_F64 ? P64 : P32)) continue;

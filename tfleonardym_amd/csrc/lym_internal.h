@@ -220,5 +220,12 @@ constexpr int kTileHifThreads64 = 320;  // complex128, HIF 2
 constexpr int64_t kTileMaxLds = 150 * 1024;
 constexpr int64_t kTileMaxLdsMr = 64 * 1024;  // multi-RHS tile
 inline int tile_hif_threads(int prec) { return prec == P64 ? kTileHifThreads64 : kTileHifThreads32; }
+// adjoint representation (matrix spinors, Ns x Nc x Nc components): its site
+// arithmetic holds ~220 (complex64) / ~370 (complex128) VGPRs, so adjoint
+// tiles are blocks of at most 256 sites with one hop in flight
+constexpr int kTileAdjThreads = 256;
+inline int tile_max_threads(int adj, int prec) {
+  return adj ? kTileAdjThreads : (prec == P64 ? kTileMaxThreads64 : kTileMaxThreads);
+}
 
 }  // namespace lym

@@ -442,7 +442,8 @@ __global__ __launch_bounds__(kBlock) void k_wilson(Lat lat, SiteMap map, SpinorV
 }
 
 // ---------------------------------------------------------------------------
-// LDS-tiled kernel (4D, complex128/complex64): a block owns a tile of
+// LDS-tiled kernel (4D, complex128/complex64; fundamental and adjoint
+// spinors, the adjoint with one hop in flight): a block owns a tile of
 // a0 x a1 x k x T sites (T = L_3, whole lines), one site per thread, thread
 // id = ((t0 * a1 + t1) * k + t2) * T + t3.  The block first copies the input
 // spinor of its tile into LDS (one coalesced read per site), then every hop
@@ -523,9 +524,10 @@ struct TileView {
 };
 
 template <int NC, bool ADJ, int P, class Epi, int HIF>
-__global__ __launch_bounds__(HIF == 1 ? (P == P64 ? kTileMaxThreads64 : kTileMaxThreads)
-                                       : (P == P64 ? kTileHifThreads64 : kTileHifThreads32),
-                             HIF == 1 ? 3 : 2) void k_wilson_tile(Lat lat, TileMap tm, SpinorView<P> in,
+__global__ __launch_bounds__(ADJ ? kTileAdjThreads
+                                 : HIF == 1 ? (P == P64 ? kTileMaxThreads64 : kTileMaxThreads)
+                                            : (P == P64 ? kTileHifThreads64 : kTileHifThreads32),
+                             ADJ ? 1 : HIF == 1 ? 3 : 2) void k_wilson_tile(Lat lat, TileMap tm, SpinorView<P> in,
                                                                  int64_t in_stride,
                                                                  LinkView<typename Store<P>::R> lk,
                                                                  LinkView<typename Store<P>::R> lkr, Epi epi,
@@ -781,6 +783,27 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
       return LYM_OK;
     }
   }
+  if constexpr (D == 4 && ADJ) {
+    // adjoint: the same tile kernel with one hop in flight (register budget)
+    if (proj && wilson_tiled(d, P)) {
+      const TileMap& tm = d.tile[P == P64 ? 0 : 1];
+      const size_t lds = (size_t)NCOMP * tm.nsite * sizeof(typename Store<P>::lds_elem);
+      const dim3 grid((unsigned)tm.blocks * (unsigned)nrhs), block(tm.nsite);
+      if (tm.nsite > kTileAdjThreads) {
+        set_error("adjoint LDS tile larger than its block limit");
+        return LYM_ERR_ARG;
+      }
+      if (lds > 64 * 1024) {
+        static std::atomic<uint32_t> raised;
+        const int rc = raise_lds_limit((const void*)k_wilson_tile<NC, ADJ, P, Epi, 1>, (int)kTileMaxLds, raised);
+        if (rc) return rc;
+      }
+      hipLaunchKernelGGL((k_wilson_tile<NC, ADJ, P, Epi, 1>), grid, block, lds, stream, d.lat, tm, sv, a.rhs_stride,
+                         lk, lkr, epi, rc, nrhs);
+      LYM_HIP(hipGetLastError());
+      return LYM_OK;
+    }
+  }
   if constexpr (D == 4 && !ADJ) {
     if (proj && wilson_tiled(d, P)) {
       const TileMap& tm = d.tile[P == P64 ? 0 : 1];
@@ -848,19 +871,20 @@ static int launch_geom(const OpDesc& d, bool proj, Field in, const EpiArgs& a, R
         set_error("half-precision storage: fundamental representation only");
         return LYM_ERR_UNSUPPORTED;
     }
-  }
-  switch (key) {
-    case 420: return launch_one<4, 2, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
-    case 430: return launch_one<4, 3, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
-    case 220: return launch_one<2, 2, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
-    case 230: return launch_one<2, 3, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
-    case 421: return launch_one<4, 2, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
-    case 431: return launch_one<4, 3, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
-    case 221: return launch_one<2, 2, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
-    case 231: return launch_one<2, 3, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
-    default:
-      set_error("unsupported (ndim, nc, representation)");
-      return LYM_ERR_UNSUPPORTED;
+  } else {
+    switch (key) {
+      case 420: return launch_one<4, 2, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      case 430: return launch_one<4, 3, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      case 220: return launch_one<2, 2, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      case 230: return launch_one<2, 3, false, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      case 421: return launch_one<4, 2, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      case 431: return launch_one<4, 3, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      case 221: return launch_one<2, 2, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      case 231: return launch_one<2, 3, true, P, EpiT>(d, proj, in, a, rc, st, nrhs);
+      default:
+        set_error("unsupported (ndim, nc, representation)");
+        return LYM_ERR_UNSUPPORTED;
+    }
   }
 }
 
