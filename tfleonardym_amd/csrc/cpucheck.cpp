@@ -7,6 +7,7 @@
 #include <type_traits>
 
 #include "force_site.h"
+#include "lym_fin.h"
 #include "wilson_site.h"
 
 using namespace lym;
@@ -182,6 +183,30 @@ extern "C" int lymcheck_wilson_apply_half(int ndim, int nc, int herm, const int*
 }
 
 // the buffer-range predicate of the host argument checks (lym_types.h)
+// The deferred multishift update as the shift-update kernel applies it
+// (fold_shift_block, then x += A p + sum_j d_j r_j, p = B p + sum_j e_j r_j),
+// on n complex elements; r holds nb >= mk residual vectors, steps j >= mk
+// contribute nothing (a shift that converged inside the block).
+extern "C" int lymcheck_fold_shift_block(int mk, int nb, const double* a, const double* z, const double* b, int64_t n,
+                                         const double* r, double* x, double* p) {
+  if (mk < 0 || nb < mk || nb > kMaxBlock) return 1;
+  double2 de[kMaxBlock];
+  double A = 0.0, B = 1.0;
+  fold_shift_block(a, z, b, mk, de, 1, A, B);
+  for (int j = mk; j < nb; ++j) de[j] = make_double2(0.0, 0.0);
+  for (int64_t i = 0; i < 2 * n; ++i) {
+    double xv = fma(A, p[i], x[i]);
+    double pv = p[i] * B;
+    for (int j = 0; j < nb; ++j) {
+      xv = fma(de[j].x, r[(int64_t)j * 2 * n + i], xv);
+      pv = fma(de[j].y, r[(int64_t)j * 2 * n + i], pv);
+    }
+    x[i] = xv;
+    p[i] = pv;
+  }
+  return 0;
+}
+
 extern "C" int cpucheck_range_inside(uint64_t base, uint64_t size, uint64_t p, uint64_t bytes) {
   return range_inside((uintptr_t)base, (size_t)size, (uintptr_t)p, (size_t)bytes) ? 1 : 0;
 }

@@ -2,6 +2,7 @@
 // finalizer kernel after it has summed the partials of the preceding
 // reducing kernel.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <math.h>
 
 #include "lym_state.h"
@@ -18,6 +19,31 @@ struct ShiftIn {
   double delta[kMaxShifts], tol2_s[kMaxShifts];
   int active[kMaxShifts];
 };
+
+// A block of deferred multishift steps folded into one update.  Steps
+// j < mk of the CG-M shift recurrence (r_j the base residual of step j)
+//   x += a_j p ;  p = z_j r_j + b_j p
+// amount to
+//   x <- x + A p + sum_j d_j r_j ,   p <- B p + sum_j e_j r_j
+// with, for j < mk,
+//   d_j = z_j S_j ,  e_j = z_j prod_{l > j} b_l ,
+//   S_j = sum_{i > j} a_i prod_{j < l < i} b_l   (S_{mk-1} = 0, S_{j-1} = a_j + b_j S_j),
+//   A = S_{-1} = sum_i a_i prod_{l < i} b_l ,  B = prod_l b_l .
+// The residuals of a CG recurrence are mutually orthogonal and p before the
+// block is orthogonal to all of them, so the terms of each sum are
+// orthogonal and no cancellation enters the FP64 accumulation.  Writes
+// de[j * stride] = (d_j, e_j) for j < mk.
+__host__ __device__ inline void fold_shift_block(const double* a, const double* z, const double* b, int mk,
+                                                 double2* de, int stride, double& A, double& B) {
+  double S = 0.0, Gp = 1.0;
+  for (int j = mk - 1; j >= 0; --j) {
+    de[j * stride] = make_double2(z[j] * S, z[j] * Gp);
+    S = fma(b[j], S, a[j]);
+    Gp *= b[j];
+  }
+  A = S;
+  B = Gp;
+}
 
 // Every thread of the block calls this (one barrier); kMaxShifts <= block size.
 __device__ inline void stage_shifts(const SolverState* st, ShiftIn& in) {

@@ -9,7 +9,7 @@
 namespace lym {
 
 constexpr int kMaxShifts = 64;
-constexpr int kMaxBlock = 64;  // deferred multishift updates: iterations per block
+constexpr int kMaxBlock = 128;  // deferred multishift updates: iterations per block
 
 enum Phase : int {
   PH_K1 = 0,    // t = Q p

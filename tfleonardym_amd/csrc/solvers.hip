@@ -339,42 +339,48 @@ __global__ __launch_bounds__(kBlock) void k_fast_kp16(const SolverState* st, int
   }
 }
 
-// Deferred shift updates of one block: for every shift, apply its recorded
-// steps j < n_upd[s]:  x_s += a_j p_s ;  p_s = z_j r^{k0+j+1} + b_j p_s
-// with x_s, p_s in FP64.  One thread per element (component plane c, site);
-// the block's residuals r^{k0+1..k0+nb} at that element are loaded once into
-// registers (complex128, or complex64 / 16-bit decoded to float, a 16-bit
-// ring's site norm multiplying the integers), then the active shifts stream
-// their x_s, p_s through them, the next shift's pair loaded while the current
-// one is updated.  Per block the kernel moves nb residuals plus 4 vectors per
-// active shift, so the shift traffic per iteration falls with the block
-// length MB (16 for the FP64 ring, 64 VGPRs; 32 for the float ring, 64
-// VGPRs).  The workgroup first copies the active shifts' step coefficients
-// (a, z, b) and the list of active shifts into LDS: read per wave from the
-// scalar cache they did not fit it (24 shifts x 32 steps x 24 B = 18 KiB)
-// and the kernel stalled on scalar loads at 1.3-2.4 TB/s.
+// Deferred shift updates of one block: every active shift applies its
+// recorded steps j < n_upd[s] (x_s += a_j p_s ; p_s = z_j r^{k0+j+1} + b_j p_s,
+// x_s and p_s in FP64) in the folded form of fold_shift_block:
+//   x_s += A_s p_s + sum_j d_sj r_j ,  p_s = B_s p_s + sum_j e_sj r_j .
+// One thread per element (component plane c, site) holds x_s, p_s of a
+// group of up to kK3Group shifts in registers and streams the block's
+// residuals through them once (complex128, or complex64 / 16-bit widened to
+// FP64, a 16-bit ring's site norm multiplying the integers); grid.z runs the
+// groups, each reading the residuals again.  Per block the kernel moves the
+// nb residuals once per group and 4 FP64 vectors per active shift, so the
+// shift traffic per iteration falls as 1/nb with the block length, which the
+// ring's HBM footprint (nb + 1 slots) bounds rather than registers.  The
+// workgroup first folds its shifts' recorded steps into LDS (one thread per
+// shift, the recurrence of fold_shift_block), as (d, e) pairs step-major so
+// one 16-byte LDS read serves a shift's two FMAs per component.
 // Work item m covers site 64 * (m / 64 / ncomp) + m % 64 of plane
 // (m / 64) % ncomp: consecutive waves take the planes of the same 64 sites,
 // so the per-site norms of a 16-bit ring are re-read from cache.
-constexpr int kK3Block64 = 16;
-constexpr int kK3BlockLo = 32;
+constexpr int kK3Block64 = 64;
+constexpr int kK3BlockLo = 128;
+constexpr int kK3Group = 12;   // shifts per thread: x_s, p_s take 8 VGPRs each
+constexpr int kK3Unroll = 2;   // residuals in flight per thread
+
+// groups of at most kK3Group shifts, as even as possible
+__host__ __device__ inline int k3_groups(int nact) { return (nact + kK3Group - 1) / kK3Group; }
 
 template <int PR>
 __global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int ncomp, int64_t V, Field ring,
                                                     int64_t ring_stride, int nslot, const Field* xs, const Field* ps,
                                                     int nshift) {
-  constexpr int MB = PR == P64 ? kK3Block64 : kK3BlockLo;
-  using RR = std::conditional_t<PR == P64, double, float>;
+  constexpr int G = kK3Group, U = kK3Unroll;
   using E = Elem<PR>;
-  // LDS (dynamic, k3_lds_bytes): coefficients [active shift][step] and the
-  // active shifts' indices and step counts
+  // LDS (dynamic, k3_lds_bytes): (d, e) [step][group slot], (A, B) per
+  // group slot, the active shifts' indices and step counts
   extern __shared__ __align__(16) unsigned char k3_smem[];
-  double(*sa)[MB] = reinterpret_cast<double(*)[MB]>(k3_smem);
-  double(*sz)[MB] = sa + nshift;
-  double(*sb)[MB] = sz + nshift;
-  int* sk = reinterpret_cast<int*>(sb + nshift);
+  const int mbmax = nslot - 1;
+  double2* sde = reinterpret_cast<double2*>(k3_smem);
+  double2* sab = sde + (size_t)mbmax * G;
+  int* sk = reinterpret_cast<int*>(sab + G);
   int* sm = sk + nshift;
   __shared__ int snact;
+  __shared__ double2* sxp[2 * G];
   const int rhs = blockIdx.y;
   st += rhs;
   if (st->phase != PH_K3) return;
@@ -390,20 +396,29 @@ __global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int n
   }
   __syncthreads();
   const int nact = snact;
-  for (int t = threadIdx.x; t < nact * MB; t += blockDim.x) {
-    const int q = t / MB, j = t - q * MB;
-    const int k = sk[q];
-    sa[q][j] = st->h_alpha[k][j];
-    sz[q][j] = st->h_zeta[k][j];
-    sb[q][j] = st->h_beta[k][j];
+  const int ngrp = k3_groups(nact);
+  if ((int)blockIdx.z >= ngrp) return;
+  const int per = (nact + ngrp - 1) / ngrp;
+  const int g0 = (int)blockIdx.z * per;
+  const int ng = min(per, nact - g0);
+  const int nb = st->blk_j;
+  if ((int)threadIdx.x < ng) {
+    const int g = (int)threadIdx.x;
+    const int k = sk[g0 + g], mk = sm[g0 + g];
+    double A, B;
+    fold_shift_block(st->h_alpha[k], st->h_zeta[k], st->h_beta[k], mk, sde + g, G, A, B);
+    for (int j = mk; j < nb; ++j) sde[j * G + g] = make_double2(0.0, 0.0);
+    sab[g] = make_double2(A, B);
+    sxp[g] = (double2*)xs[rhs * nshift + k].data;
+    sxp[G + g] = (double2*)ps[rhs * nshift + k].data;
   }
   __syncthreads();
-  if (nact == 0) return;
   const E* rg = (const E*)ring.data + rhs * ring_stride;
   const float* rn = ring.norm ? ring.norm + (int64_t)rhs * nslot * V : nullptr;
-  xs += rhs * nshift;
-  ps += rhs * nshift;
-  const int nb = st->blk_j;
+  // the group's field pointers, read from LDS at every element (held in
+  // scalar registers they spill)
+  double2** xp = sxp;
+  double2** pp = sxp + G;
   const int k0 = st->iter - nb;
   const int64_t n = (int64_t)ncomp * V;
   const int64_t nwork = ((V + 63) / 64) * ncomp * 64;
@@ -413,65 +428,70 @@ __global__ __launch_bounds__(kBlock) void k_fast_k3(const SolverState* st, int n
     const int64_t site = (grp / ncomp) * 64 + (m & 63);
     if (site >= V) continue;
     const int64_t i = (int64_t)c * V + site;
-    RR rr[MB], ri[MB];
+    const int64_t ie = PR == P16 ? p16_index(ncomp, c, V, site) : i;
+    double2 x[G], p[G];
 #pragma unroll
-    for (int j = 0; j < MB; ++j) {
-      rr[j] = RR(0);
-      ri[j] = RR(0);
-      if (j < nb) {
-        const int q = (k0 + j + 1) % nslot;
-        const E e = rg[(int64_t)q * n + (PR == P16 ? p16_index(ncomp, c, V, site) : i)];
-        if constexpr (PR == P16) {
-          const float sc = rn[(int64_t)q * V + site];
-          rr[j] = (float)e.x * sc;
-          ri[j] = (float)e.y * sc;
-        } else {
-          rr[j] = e.x;
-          ri[j] = e.y;
+    for (int g = 0; g < G; ++g)
+      if (g < ng) {
+        x[g] = xp[g][i];
+        p[g] = pp[g][i];
+      }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (g < ng) {
+        const double2 ab = sab[g];
+        x[g].x = fmad(ab.x, p[g].x, x[g].x);
+        x[g].y = fmad(ab.x, p[g].y, x[g].y);
+        p[g].x *= ab.y;
+        p[g].y *= ab.y;
+      }
+    for (int j0 = 0; j0 < nb; j0 += U) {
+      double rr[U], ri[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        rr[u] = 0.0;
+        ri[u] = 0.0;
+        if (j0 + u < nb) {
+          const int q = (k0 + j0 + u + 1) % nslot;
+          const E e = rg[(int64_t)q * n + ie];
+          if constexpr (PR == P16) {
+            const float sc = rn[(int64_t)q * V + site];
+            rr[u] = (double)((float)e.x * sc);
+            ri[u] = (double)((float)e.y * sc);
+          } else {
+            rr[u] = (double)e.x;
+            ri[u] = (double)e.y;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (j0 + u < nb) {
+          const double2* cj = sde + (j0 + u) * G;
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+            if (g < ng) {
+              const double2 de = cj[g];
+              x[g].x = fmad(de.x, rr[u], x[g].x);
+              x[g].y = fmad(de.x, ri[u], x[g].y);
+              p[g].x = fmad(de.y, rr[u], p[g].x);
+              p[g].y = fmad(de.y, ri[u], p[g].y);
+            }
         }
       }
     }
-    int k = sk[0];
-    double2 x = ((const double2*)xs[k].data)[i];
-    double2 p = ((const double2*)ps[k].data)[i];
-    for (int qa = 0; qa < nact; ++qa) {
-      double2 xn = x, pn = p;
-      int kn = -1;
-      if (qa + 1 < nact) {  // the next shift's pair is in flight during this update
-        kn = sk[qa + 1];
-        xn = ((const double2*)xs[kn].data)[i];
-        pn = ((const double2*)ps[kn].data)[i];
-      }
-      if constexpr (PR != P64) {
-        // converted per use: otherwise the compiler hoists the conversions
-        // out of the shift loop and keeps 4 MB doubles live instead of 2 MB floats
 #pragma unroll
-        for (int j = 0; j < MB; ++j) asm volatile("" : "+v"(rr[j]), "+v"(ri[j]));
+    for (int g = 0; g < G; ++g)
+      if (g < ng) {
+        xp[g][i] = x[g];
+        pp[g][i] = p[g];
       }
-      const int mk = sm[qa];
-#pragma unroll
-      for (int j = 0; j < MB; ++j) {
-        if (j < mk) {
-          const double a = sa[qa][j], z = sz[qa][j], b = sb[qa][j];
-          x.x = fmad(a, p.x, x.x);
-          x.y = fmad(a, p.y, x.y);
-          p.x = fmad(z, (double)rr[j], b * p.x);
-          p.y = fmad(z, (double)ri[j], b * p.y);
-        }
-      }
-      ((double2*)xs[k].data)[i] = x;
-      ((double2*)ps[k].data)[i] = p;
-      k = kn;
-      x = xn;
-      p = pn;
-    }
   }
 }
 
-// dynamic LDS of k_fast_k3 for `nshift` shifts
-static size_t k3_lds_bytes(int prec, int nshift) {
-  const int mb = prec == P64 ? kK3Block64 : kK3BlockLo;
-  return (size_t)nshift * (3 * mb * sizeof(double) + 2 * sizeof(int));
+// dynamic LDS of k_fast_k3 for a ring of `nslot` slots and `nshift` shifts
+static size_t k3_lds_bytes(int nslot, int nshift) {
+  return (size_t)((nslot - 1) * kK3Group + kK3Group) * sizeof(double2) + (size_t)2 * nshift * sizeof(int);
 }
 
 // Shared state transition of a precision switch (block (0, rhs), thread 0):
@@ -488,44 +508,62 @@ __device__ inline void fast_switch_state(SolverState* st, BatchCtl* ctl, int mbl
   if (blockIdx.y == 0) ctl->nswitch = ctl->ndone;
 }
 
-// precision switch FP64 -> complex64 at a block boundary: copies of the
-// current residual (FP64 ring slot iter % nslot64 -> low ring slot
-// iter % nslot_lo) and of the live direction; later blocks run mblk_lo
-// iterations
+// precision switch, before the new ring is written: every RHS still in a
+// block (PH_KP) has it flushed by the shift update that follows
+__global__ void k_fast_arm_flush(SolverState* st) {
+  if (threadIdx.x != 0) return;
+  st += blockIdx.y;
+  if (st->phase == PH_KP) st->phase = PH_K3;
+}
+
+// precision switch: the current residual (ring slot iter % nslot) copied to
+// the old phase's t field, n elements per RHS; the rings of the phases share
+// memory, so the new ring's slot may overlap the old one's
+template <typename E>
+__global__ __launch_bounds__(kBlock) void k_fast_stage(const SolverState* st, int64_t n, const E* __restrict__ ring,
+                                                       int64_t ring_stride, int nslot, E* __restrict__ t) {
+  const int rhs = blockIdx.y;
+  st += rhs;
+  ring += rhs * ring_stride + (int64_t)(st->iter % nslot) * n;
+  t += rhs * n;
+  LYM_GRID_STRIDE(i, n) t[i] = ring[i];
+}
+
+// precision switch FP64 -> complex64 after the flush: the staged residual
+// r64 (n elements per RHS) goes to low ring slot iter % nslot_lo and the live
+// direction is converted; later blocks run mblk_lo iterations
 template <int PL>
 __global__ __launch_bounds__(kBlock) void k_fast_switch(SolverState* st, BatchCtl* ctl, int64_t n,
-                                                        const double2* __restrict__ ring64, int64_t ring_stride64,
-                                                        int nslot64, Elem<PL>* __restrict__ ringlo,
+                                                        const double2* __restrict__ r64, Elem<PL>* __restrict__ ringlo,
                                                         int64_t ring_stride_lo, int nslot_lo,
                                                         const double2* __restrict__ p64, Elem<PL>* __restrict__ plo,
                                                         int mblk_lo) {
   const int rhs = blockIdx.y;
   st += rhs;
-  ring64 += rhs * ring_stride64;
+  r64 += rhs * n;
   ringlo += rhs * ring_stride_lo;
   p64 += rhs * n;
   plo += rhs * n;
-  const int64_t off64 = (int64_t)(st->iter % nslot64) * n;
   const int64_t offlo = (int64_t)(st->iter % nslot_lo) * n;
   LYM_GRID_STRIDE(i, n) {
-    ringlo[offlo + i] = from64<Elem<PL>>(ring64[off64 + i]);
+    ringlo[offlo + i] = from64<Elem<PL>>(r64[i]);
     plo[i] = from64<Elem<PL>>(p64[i]);
   }
   fast_switch_state(st, ctl, mblk_lo);
 }
 
-// precision switch complex64 -> 16-bit storage (one thread per site)
+// precision switch complex64 -> 16-bit storage after the flush (one thread
+// per site): the staged residual r32 (NCOMP * V elements per RHS) goes to
+// 16-bit ring slot iter % nslot16
 template <int NCOMP>
 __global__ __launch_bounds__(kBlock) void k_fast_switch16(SolverState* st, BatchCtl* ctl, int64_t V,
-                                                          const float2* __restrict__ ring32, int nslot32,
-                                                          Field ring16, int nslot16, const float2* __restrict__ p32,
-                                                          Field p16, int mblk_lo) {
+                                                          const float2* __restrict__ r32, Field ring16, int nslot16,
+                                                          const float2* __restrict__ p32, Field p16, int mblk_lo) {
   const int rhs = blockIdx.y;
   st += rhs;
   const int64_t n = (int64_t)NCOMP * V;
-  const int64_t q32 = (int64_t)rhs * nslot32 + st->iter % nslot32;
   const int64_t q16 = (int64_t)rhs * nslot16 + st->iter % nslot16;
-  const SpinorView<P32> ri{const_cast<float2*>(ring32) + q32 * n, nullptr, V};
+  const SpinorView<P32> ri{const_cast<float2*>(r32) + rhs * n, nullptr, V};
   const SpinorView<P32> pi{const_cast<float2*>(p32) + rhs * n, nullptr, V};
   const SpinorView<P16> ro{(short2*)ring16.data + q16 * n, ring16.norm + q16 * V, V};
   const SpinorView<P16> po{(short2*)p16.data + rhs * n, p16.norm + (int64_t)rhs * V, V};
@@ -859,13 +897,11 @@ static Field take_field(Carver& c, const OpDesc& d, int prec) {
   return f;
 }
 
-// Iterations per deferred shift-update block.  A block reads its residuals
-// once and every active shift's FP64 x_s, p_s once (4 vectors per shift), so
-// the shift traffic per iteration falls as 1/m; the update kernel holds the
-// block's residuals of one element in registers (m complex numbers: 128
-// VGPRs for 32 complex128 or 64 complex64 / 16-bit ones).
-// (block lengths: CgmTuning::mblk_64 and mblk_lo, at most kMaxBlock; each
-// ring has one slot more than its block is long)
+// Iterations per deferred shift-update block (CgmTuning::mblk_64 and
+// mblk_lo, at most kMaxBlock; each ring has one slot more than its block is
+// long).  A block reads its residuals once per shift group and every active
+// shift's FP64 x_s, p_s once (4 vectors per shift), so the shift traffic per
+// iteration falls as 1/m; the ring's footprint is m + 1 fields per RHS.
 
 // Thresholds of the fast multishift CG (relative to |b|): the base residual
 // below which complex64 Krylov vectors (switch32) and 16-bit storage
@@ -882,23 +918,37 @@ static Field take_field(Carver& c, const OpDesc& d, int prec) {
 //   3e-4, 1e-7           1.059                5.57
 //   3e-4, 1e-6           1.173                5.72
 //   1e-3, 1e-6           0.866 (8 corrections, 519 it)   5.77
-// LYM_CGM_TUNE="switch32,switch16,iter_factor" overrides them (tuning runs).
+// Iterations run per host chunk (one graph replay, one poll of the batch
+// state) are fewer than a block: a precision switch waits for the chunk in
+// flight, so the chunk length bounds how far a phase overruns its threshold.
+// Blocks are whole numbers of chunks; a switch flushes the block in progress.
+// LYM_CGM_TUNE="switch32,switch16,iter_factor[,mblk_64,mblk_lo]" overrides
+// the thresholds and block lengths (tuning runs).
 struct CgmTuning {
   double switch32 = 3e-4;
   double switch16 = 1e-6;
   double iter_factor = 0.5;
   int mblk_lo = kK3BlockLo;  // block length of the complex64 / 16-bit phases
   int mblk_64 = kK3Block64;  // block length of the FP64 phase
+  int chunk_lo = 32;         // iterations per chunk, complex64 / 16-bit phases
+  int chunk_64 = 16;         // iterations per chunk, FP64 phase
 };
 static CgmTuning cgm_tuning() {
   CgmTuning t;
   const char* v = std::getenv("LYM_CGM_TUNE");
   if (v && *v) {
     double a = 0, b = 0, c = 0;
-    if (std::sscanf(v, "%lf,%lf,%lf", &a, &b, &c) == 3 && a > 0 && b > 0 && c > 0 && c <= 1) {
+    int m64 = 0, mlo = 0;
+    const int got = std::sscanf(v, "%lf,%lf,%lf,%d,%d", &a, &b, &c, &m64, &mlo);
+    if (got >= 3 && a > 0 && b > 0 && c > 0 && c <= 1) {
       t.switch32 = a;
       t.switch16 = b;
       t.iter_factor = c;
+    }
+    if (got == 5 && m64 >= t.chunk_64 && m64 <= kMaxBlock && m64 % t.chunk_64 == 0 && mlo >= t.chunk_lo &&
+        mlo <= kMaxBlock && mlo % t.chunk_lo == 0) {
+      t.mblk_64 = m64;
+      t.mblk_lo = mlo;
     }
   }
   return t;
@@ -978,8 +1028,17 @@ static size_t carve(const OpDesc& d, int prec, int nshift, int nrhs, char* base,
       const CgmTuning tun = cgm_tuning();
       P.nslot64 = tun.mblk_64 + 1;
       P.nslot_lo = tun.mblk_lo + 1;
-      P.ring64.data = c.take(field_bytes(d, P64) * P.nslot64 * nrhs);
-      P.ring32.data = c.take(field_bytes(d, P32) * P.nslot_lo * nrhs);
+      // the rings of the three phases are used one after the other and share
+      // one allocation (a precision switch stages the current residual in the
+      // old phase's t field before writing it to the new ring)
+      const size_t nbv = (size_t)d.lat.V * sizeof(float);
+      const size_t b64 = field_bytes(d, P64) * P.nslot64 * nrhs;
+      const size_t b32 = field_bytes(d, P32) * P.nslot_lo * nrhs;
+      const size_t b16d = field_bytes(d, P16) * P.nslot_lo * nrhs;
+      const size_t b16 = prec == P16 ? b16d + nbv * P.nslot_lo * nrhs : 0;
+      char* ring = (char*)c.take(std::max(b64, std::max(b32, b16)));
+      P.ring64.data = ring;
+      P.ring32.data = ring;
       P.plive64.data = c.take(field_bytes(d, P64) * nrhs);
       P.plive32.data = c.take(field_bytes(d, P32) * nrhs);
       P.tb64.data = c.take(field_bytes(d, P64) * nrhs);
@@ -987,13 +1046,12 @@ static size_t carve(const OpDesc& d, int prec, int nshift, int nrhs, char* base,
       P.ps64.data = c.take(field_bytes(d, P64) * nshift * nrhs);
       if (prec == P16) {
         // third phase: 16-bit storage with per-site norms
-        const size_t nb = (size_t)d.lat.V * sizeof(float);
-        P.ring16.data = c.take(field_bytes(d, P16) * P.nslot_lo * nrhs);
-        P.ring16.norm = (float*)c.take(nb * P.nslot_lo * nrhs);
+        P.ring16.data = ring;
+        P.ring16.norm = ring ? (float*)(ring + b16d) : nullptr;
         P.plive16.data = c.take(field_bytes(d, P16) * nrhs);
-        P.plive16.norm = (float*)c.take(nb * nrhs);
+        P.plive16.norm = (float*)c.take(nbv * nrhs);
         P.tb16.data = c.take(field_bytes(d, P16) * nrhs);
-        P.tb16.norm = (float*)c.take(nb * nrhs);
+        P.tb16.norm = (float*)c.take(nbv * nrhs);
       }
     }
   }
@@ -1360,9 +1418,10 @@ static int kt_collect(G&& shift_bytes) {
     float ms = 0.f;
     LYM_HIP(hipEventElapsedTime(&ms, r.e0, r.e1));
     KClass& c = g_kt.cls[r.cls];
-    c.launches += 1;
+    const double b = r.bytes >= 0 ? r.bytes : shift_bytes(r.chunk);
+    if (r.bytes >= 0 || b > 0) c.launches += 1;  // a shift update with no block due is not counted
     c.us += 1e3 * ms;
-    c.bytes += r.bytes >= 0 ? r.bytes : shift_bytes(r.chunk);
+    c.bytes += b;
   }
   g_kt.recs.clear();
   g_kt.used = 0;
@@ -1480,21 +1539,48 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   };
 
   int prec = P64;
-  // chunks issued so far (precision and length), for the kernel timing
-  std::vector<std::pair<int, int>> chunk_log;
+  // for the kernel timing (no graphs then, so the chunk body runs at every
+  // issue): chunks issued so far (precision and length), and every shift
+  // update launch in issue order (chunk c's end: (0, c); the flush before
+  // chunk c at a precision switch: (1, c))
+  std::vector<std::pair<int, int>> chunk_log, k3_log;
+  // deferred shift update of the current block (a no-op for RHS not in PH_K3)
+  auto shift_update = [&](hipStream_t st, int kind) -> int {
+    const Field ring = prec == P64 ? P.ring64 : prec == P32 ? P.ring32 : P.ring16;
+    const int nslot = prec == P64 ? nslot64 : prec == P32 ? nslot32 : nslot16;
+    const int64_t ring_stride = prec == P64 ? ring_stride64 : prec == P32 ? ring_stride32 : ring_stride16;
+    const int ei = (int)k3_log.size();
+    k3_log.emplace_back(kind, (int)chunk_log.size() - (kind == 0 ? 1 : 0));
+    const dim3 grid(gf, nrhs, k3_groups(nshift));
+    const size_t lds = k3_lds_bytes(nslot, nshift);
+    int e = kt_launch(st, (std::string("shift_update_") + pname(prec)).c_str(), -1.0, ei, [&]() {
+      if (prec == P64)
+        hipLaunchKernelGGL(k_fast_k3<P64>, grid, dim3(kBlock), lds, st, P.st, d.ncomp, V, ring, ring_stride, nslot,
+                           P.xs_dev, P.ps_dev, nshift);
+      else if (prec == P32)
+        hipLaunchKernelGGL(k_fast_k3<P32>, grid, dim3(kBlock), lds, st, P.st, d.ncomp, V, ring, ring_stride, nslot,
+                           P.xs_dev, P.ps_dev, nshift);
+      else
+        hipLaunchKernelGGL(k_fast_k3<P16>, grid, dim3(kBlock), lds, st, P.st, d.ncomp, V, ring, ring_stride, nslot,
+                           P.xs_dev, P.ps_dev, nshift);
+      LYM_HIP(hipGetLastError());
+      return (int)LYM_OK;
+    });
+    if (e) return e;
+    return launch_fin(FK_FAST_K3ADV, P.st, nullptr, 0, 0, nullptr, st, nrhs, P.ctl);
+  };
   auto chunk = [&](hipStream_t st) -> int {
     const Field ring = prec == P64 ? P.ring64 : prec == P32 ? P.ring32 : P.ring16;
     const Field pl = prec == P64 ? P.plive64 : prec == P32 ? P.plive32 : P.plive16;
     const Field t = prec == P64 ? P.tb64 : prec == P32 ? P.tb32 : P.tb16;
     const int gw = wilson_blocks(d, prec, nrhs);  // Wilson partials per RHS at this precision
-    const int mb = prec == P64 ? m64 : mlo;
+    const int cl = prec == P64 ? tun.chunk_64 : tun.chunk_lo;
     const int nslot = prec == P64 ? nslot64 : prec == P32 ? nslot32 : nslot16;
     const int64_t ring_stride = prec == P64 ? ring_stride64 : prec == P32 ? ring_stride32 : ring_stride16;
-    const int ci = (int)chunk_log.size();
-    chunk_log.emplace_back(prec, mb);
+    chunk_log.emplace_back(prec, cl);
     const double sites = (double)V * nrhs;
     const std::string sfx = pname(prec);
-    for (int j = 0; j < mb; ++j) {
+    for (int j = 0; j < cl; ++j) {
       int e = kt_launch(st, ("direction_update_" + sfx).c_str(), 3.0 * vec_bytes_site(d, prec) * sites, -1, [&]() {
         if (prec == P64)
           hipLaunchKernelGGL(k_fast_kp<P64>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n, (const double2*)ring.data,
@@ -1538,21 +1624,9 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
                       [&]() { return launch_fin(FK_CGM_BF, P.st, P.partials, gw, 1, nullptr, st, nrhs, P.ctl); });
       if (e) return e;
     }
-    int e = kt_launch(st, ("shift_update_" + sfx).c_str(), -1.0, ci, [&]() {
-      if (prec == P64)
-        hipLaunchKernelGGL(k_fast_k3<P64>, dim3(gf, nrhs), dim3(kBlock), k3_lds_bytes(P64, nshift), st, P.st, d.ncomp, V, ring, ring_stride,
-                           nslot, P.xs_dev, P.ps_dev, nshift);
-      else if (prec == P32)
-        hipLaunchKernelGGL(k_fast_k3<P32>, dim3(gf, nrhs), dim3(kBlock), k3_lds_bytes(P32, nshift), st, P.st, d.ncomp, V, ring, ring_stride,
-                           nslot, P.xs_dev, P.ps_dev, nshift);
-      else
-        hipLaunchKernelGGL(k_fast_k3<P16>, dim3(gf, nrhs), dim3(kBlock), k3_lds_bytes(P16, nshift), st, P.st, d.ncomp, V, ring, ring_stride,
-                           nslot, P.xs_dev, P.ps_dev, nshift);
-      LYM_HIP(hipGetLastError());
-      return (int)LYM_OK;
-    });
-    if (e) return e;
-    return launch_fin(FK_FAST_K3ADV, P.st, nullptr, 0, 0, nullptr, st, nrhs, P.ctl);
+    // the block's shift update runs at the chunk end that completes it (or
+    // where the solve stops); elsewhere the launch finds no RHS in PH_K3
+    return shift_update(st, 0);
   };
   // poll the batch control block: (ndone, nswitch, nrhs, -).  Each in-flight
   // chunk remembers the precision it ran at: a tally read from a chunk issued
@@ -1587,17 +1661,28 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
       return LYM_ERR_HIP;
     }
     if (s4[1] >= nrhs && slot_prec[slot] == prec && prec != lowprec) {
+      // the block in progress is flushed at the old precision; the new ring
+      // shares the old ring's memory, so the current residual is staged in
+      // the old phase's t = Q p (recomputed before it is read again)
+      hipLaunchKernelGGL(k_fast_arm_flush, dim3(1, nrhs), dim3(64), 0, st, P.st);
+      LYM_HIP(hipGetLastError());
+      if ((rc = shift_update(st, 1))) return rc;
       if (prec == P64) {
+        hipLaunchKernelGGL(k_fast_stage<double2>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
+                           (const double2*)P.ring64.data, ring_stride64, nslot64, (double2*)P.tb64.data);
         hipLaunchKernelGGL(k_fast_switch<P32>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, P.ctl, n,
-                           (const double2*)P.ring64.data, ring_stride64, nslot64, (float2*)P.ring32.data,
-                           ring_stride32, nslot32, (const double2*)P.plive64.data, (float2*)P.plive32.data, mlo);
+                           (const double2*)P.tb64.data, (float2*)P.ring32.data, ring_stride32, nslot32,
+                           (const double2*)P.plive64.data, (float2*)P.plive32.data, mlo);
         LYM_HIP(hipGetLastError());
         prec = P32;
       } else {
+        hipLaunchKernelGGL(k_fast_stage<float2>, dim3(gf, nrhs), dim3(kBlock), 0, st, P.st, n,
+                           (const float2*)P.ring32.data, ring_stride32, nslot32, (float2*)P.tb32.data);
+        LYM_HIP(hipGetLastError());
         int e = by_ncomp([&](auto nc) {
           hipLaunchKernelGGL(k_fast_switch16<decltype(nc)::value>, dim3(gs, nrhs), dim3(kBlock), 0, st, P.st, P.ctl,
-                             V, (const float2*)P.ring32.data, nslot32, P.ring16, nslot16,
-                             (const float2*)P.plive32.data, P.plive16, mlo);
+                             V, (const float2*)P.tb32.data, P.ring16, nslot16, (const float2*)P.plive32.data,
+                             P.plive16, mlo);
         });
         if (e) return e;
         prec = P16;
@@ -1610,24 +1695,42 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   LYM_HIP(hipMemcpyAsync(out_states.data(), P.st, sizeof(SolverState) * nrhs, hipMemcpyDeviceToHost, st));
   LYM_HIP(hipStreamSynchronize(st));
   if (g_kt.on) {
-    // first iteration of every chunk, then the shift-update bytes of chunk c:
-    // per RHS, the block's residuals (nb of them) and 4 FP64 vectors of
-    // every shift still active at its start
+    // bytes of every shift update launch, replaying the block boundaries of
+    // each RHS: chunk c covers iterations k0[c] .. k0[c + 1]; a phase's blocks
+    // end every mblk iterations from its first one, where the RHS stops, and
+    // at the flush before a precision switch.  A block of nb steps reads its
+    // residuals once per shift group and 4 FP64 vectors of every shift
+    // active at its start.
     std::vector<int> k0(chunk_log.size() + 1, 0);
     for (size_t c = 0; c < chunk_log.size(); ++c) k0[c + 1] = k0[c] + chunk_log[c].second;
-    rc = kt_collect([&](int c) {
-      double b = 0.0;
-      for (int r = 0; r < nrhs; ++r) {
-        const SolverState& h = out_states[r];
-        const int nb = std::min(chunk_log[c].second, h.iter - k0[c]);
+    std::vector<double> ev_bytes(k3_log.size(), 0.0);
+    for (int r = 0; r < nrhs; ++r) {
+      const SolverState& h = out_states[r];
+      int f = 0, sp = 0, mb = m64;
+      for (size_t e = 0; e < k3_log.size(); ++e) {
+        const int kind = k3_log[e].first, c = k3_log[e].second;
+        int nb = 0, pr = P64;
+        if (kind == 0) {
+          pr = chunk_log[c].first;
+          const int end = k0[c + 1];
+          if (f < h.iter && h.iter <= end) nb = h.iter - f;
+          else if (f < h.iter && (end - sp) % mb == 0) nb = end - f;
+        } else {
+          pr = c > 0 ? chunk_log[c - 1].first : P64;
+          if (h.iter > k0[c]) nb = k0[c] - f;
+          sp = k0[c];
+          mb = mlo;
+        }
         if (nb <= 0) continue;
         int act = 0;
         for (int k = 0; k < nshift; ++k)
-          if (h.conv_iter[k] < 0 || h.conv_iter[k] > k0[c]) ++act;
-        b += (double)V * (nb * vec_bytes_site(d, chunk_log[c].first) + act * 4.0 * vec_bytes_site(d, P64));
+          if (h.conv_iter[k] < 0 || h.conv_iter[k] > f) ++act;
+        ev_bytes[e] += (double)V * ((double)k3_groups(act) * nb * vec_bytes_site(d, pr) +
+                                    act * 4.0 * vec_bytes_site(d, P64));
+        f += nb;
       }
-      return b;
-    });
+    }
+    rc = kt_collect([&](int e) { return ev_bytes[e]; });
     if (rc) return rc;
   }
   return LYM_OK;
