@@ -1475,6 +1475,9 @@ static double wilson_bytes(const OpDesc& d, int prec, int nvec, int nrhs) {
 // the shifted solutions at the size of the residual it is made at, so the
 // thresholds keep each phase's contribution to the true residual below the
 // target.  xs[r * nshift + s] are the caller's FP64 solutions.
+static int batched_tile(const OpDesc& d, const Parts& P, int pl, int nrhs, Field in, Field out, hipStream_t st,
+                        OpDesc& db);
+
 static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void* const* xs, int nrhs, int nshift,
                     const double* shifts, const double* tol_rel, int max_iter, const CgmTuning& tun, int lowprec,
                     hipStream_t st, std::vector<SolverState>& out_states) {
@@ -1538,6 +1541,13 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
     return LYM_OK;
   };
 
+  // Wilson tiles of the batch, per precision (tuned on the first batched
+  // solve of this shape: t = Q p timed on the phase's own fields, which hold
+  // nothing yet except the FP64 direction)
+  OpDesc d64 = d, d32 = d, d16 = d;
+  if ((rc = batched_tile(d, P, P64, nrhs, P.plive64, P.tb64, st, d64))) return rc;
+  if ((rc = batched_tile(d, P, P32, nrhs, P.plive32, P.tb32, st, d32))) return rc;
+  if (lowprec == P16 && (rc = batched_tile(d, P, P16, nrhs, P.plive16, P.tb16, st, d16))) return rc;
   int prec = P64;
   // for the kernel timing (no graphs then, so the chunk body runs at every
   // issue): chunks issued so far (precision and length), and every shift
@@ -1573,7 +1583,8 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
     const Field ring = prec == P64 ? P.ring64 : prec == P32 ? P.ring32 : P.ring16;
     const Field pl = prec == P64 ? P.plive64 : prec == P32 ? P.plive32 : P.plive16;
     const Field t = prec == P64 ? P.tb64 : prec == P32 ? P.tb32 : P.tb16;
-    const int gw = wilson_blocks(d, prec, nrhs);  // Wilson partials per RHS at this precision
+    const OpDesc& dp = prec == P64 ? d64 : prec == P32 ? d32 : d16;
+    const int gw = wilson_blocks(dp, prec, nrhs);  // Wilson partials per RHS at this precision
     const int cl = prec == P64 ? tun.chunk_64 : tun.chunk_lo;
     const int nslot = prec == P64 ? nslot64 : prec == P32 ? nslot32 : nslot16;
     const int64_t ring_stride = prec == P64 ? ring_stride64 : prec == P32 ? ring_stride32 : ring_stride16;
@@ -1604,7 +1615,7 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
       a1.st = P.st;
       a1.rhs_stride = n;
       e = kt_launch(st, ("wilson_cgm_k1_" + sfx).c_str(), wilson_bytes(d, prec, 2, nrhs), -1,
-                    [&]() { return wilson_launch(EPI_CGM_K1, d, prec, true, pl, a1, redctl(P), st, nrhs); });
+                    [&]() { return wilson_launch(EPI_CGM_K1, dp, prec, true, pl, a1, redctl(P), st, nrhs); });
       if (!e)
         e = kt_launch(st, "finalize", 0.0, -1,
                       [&]() { return launch_fin(FK_CGM_A, P.st, P.partials, gw, 2, nullptr, st, nrhs, P.ctl); });
@@ -1618,7 +1629,7 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
       a2.rhs_stride = n;
       a2.ring_rhs_stride = ring_stride;
       e = kt_launch(st, ("wilson_cgm_k2f_" + sfx).c_str(), wilson_bytes(d, prec, 4, nrhs), -1,
-                    [&]() { return wilson_launch(EPI_CGM_K2F, d, prec, true, t, a2, redctl(P), st, nrhs); });
+                    [&]() { return wilson_launch(EPI_CGM_K2F, dp, prec, true, t, a2, redctl(P), st, nrhs); });
       if (!e)
         e = kt_launch(st, "finalize", 0.0, -1,
                       [&]() { return launch_fin(FK_CGM_BF, P.st, P.partials, gw, 1, nullptr, st, nrhs, P.ctl); });
@@ -1638,9 +1649,10 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
   int slot_prec[2] = {P64, P64};
   KeyBuf kb;
   kb.add('F').add(nrhs).add(nshift).add(P.st).add(P.xs_dev).add(P.ring64.data);
-  const std::string key = op_key(d) + kb.s;
+  const std::string key64 = op_key(d64) + kb.s + 'd', key32 = op_key(d32) + kb.s + 's';
+  const std::string key16 = op_key(d16) + kb.s + 'h';
   auto issue = [&](int slot) -> int {
-    const int e = launch_chunk(key + (prec == P64 ? 'd' : prec == P32 ? 's' : 'h'), st, chunk);
+    const int e = launch_chunk(prec == P64 ? key64 : prec == P32 ? key32 : key16, st, chunk);
     if (e) return e;
     LYM_HIP(hipMemcpyAsync(&g_sync.pinned[4 * slot], P.ctl, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
     LYM_HIP(hipEventRecord(g_sync.ev[slot], st));
@@ -1772,20 +1784,23 @@ static bool batch_tune_enabled() {
   return !(v && v[0] == '1');
 }
 
-// db = d with the batched tile in d.tile[1] (unchanged when the lattice is
-// not tiled, the multi-RHS kernel runs the batch, tuning is off, or no RHS
-// is active).  Leaves P.rl_t and the
-// partials overwritten; the states are only read.
-static int batched_tile(const OpDesc& d, const Parts& P, int pl, int nrhs, hipStream_t st, OpDesc& db) {
+// db = d with the batched tile of storage precision pl in d.tile[] (unchanged
+// when the lattice is not tiled, the multi-RHS kernel runs the batch, tuning
+// is off, or no RHS is active).  The timed launches read `in` and leave
+// `out` (batched fields of precision pl) and the partials overwritten; the
+// states are only read.
+static int batched_tile(const OpDesc& d, const Parts& P, int pl, int nrhs, Field in, Field out, hipStream_t st,
+                        OpDesc& db) {
   db = d;
   if (nrhs < 2 || !wilson_tiled(d, pl) || wilson_mr(d, pl, nrhs) || d.adj || d.ndim != 4 || !batch_tune_enabled())
     return LYM_OK;
+  const int slot = pl == P64 ? 0 : 1;
   const BatchTuneKey key{d.nc, pl, nrhs, {d.lat.L[0], d.lat.L[1], d.lat.L[2], d.lat.L[3]}};
   {
     std::lock_guard<std::mutex> lock(g_btune_mu);
     auto it = g_btuned.find(key);
     if (it != g_btuned.end()) {
-      db.tile[1] = it->second;
+      db.tile[slot] = it->second;
       return LYM_OK;
     }
   }
@@ -1793,8 +1808,11 @@ static int batched_tile(const OpDesc& d, const Parts& P, int pl, int nrhs, hipSt
   LYM_HIP(hipMemcpyAsync(&c, P.ctl, sizeof(c), hipMemcpyDeviceToHost, st));
   LYM_HIP(hipStreamSynchronize(st));
   if (c.ndone >= nrhs) return LYM_OK;  // every source is zero: nothing to time
+  // LDS bytes per site and component: complex128, or complex64 (16-bit
+  // storage is decoded to complex64 in the tile)
+  const int64_t es = (int64_t)d.ncomp * (pl == P64 ? 16 : 8);
   std::vector<TileMap> cands;
-  for (const TileMap& t : tile_candidates(d, (int64_t)d.ncomp * 8, kTileMaxThreads, kTileMaxLds)) {
+  for (const TileMap& t : tile_candidates(d, es, tile_max_threads(0, pl), kTileMaxLds)) {
     for (int hif : {1, 2}) {
       if (hif > 1 && t.nsite > tile_hif_threads(pl)) continue;
       TileMap x = t;
@@ -1807,21 +1825,21 @@ static int batched_tile(const OpDesc& d, const Parts& P, int pl, int nrhs, hipSt
   LYM_HIP(hipEventCreate(&e0));
   LYM_HIP(hipEventCreate(&e1));
   EpiArgs a1;
-  a1.out = P.rl_t;
+  a1.out = out;
   a1.st = P.st;
   a1.rhs_stride = field_elems(d);
   int rc = LYM_OK;
   double best_us = 1e30;
-  TileMap best = d.tile[1];
+  TileMap best = d.tile[slot];
   const int reps = 3;
   for (size_t ci = 0; ci < cands.size() && !rc; ++ci) {
     OpDesc dc = d;
-    dc.tile[1] = cands[ci];
-    for (int i = 0; i < 2 && !rc; ++i) rc = wilson_launch(EPI_CGM_K1, dc, pl, true, P.rl_p, a1, redctl(P), st, nrhs);
+    dc.tile[slot] = cands[ci];
+    for (int i = 0; i < 2 && !rc; ++i) rc = wilson_launch(EPI_CGM_K1, dc, pl, true, in, a1, redctl(P), st, nrhs);
     double us = 1e30;
     for (int trial = 0; trial < 2 && !rc; ++trial) {
       if (hipEventRecord(e0, st) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventRecord");
-      for (int i = 0; i < reps && !rc; ++i) rc = wilson_launch(EPI_CGM_K1, dc, pl, true, P.rl_p, a1, redctl(P), st, nrhs);
+      for (int i = 0; i < reps && !rc; ++i) rc = wilson_launch(EPI_CGM_K1, dc, pl, true, in, a1, redctl(P), st, nrhs);
       if (!rc && hipEventRecord(e1, st) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventRecord");
       float ms = 0.f;
       if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = hip_fail(hipErrorUnknown, "hipEventSynchronize");
@@ -1840,7 +1858,7 @@ static int batched_tile(const OpDesc& d, const Parts& P, int pl, int nrhs, hipSt
     std::lock_guard<std::mutex> lock(g_btune_mu);
     g_btuned[key] = best;
   }
-  db.tile[1] = best;
+  db.tile[slot] = best;
   return LYM_OK;
 }
 
@@ -1901,7 +1919,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
   if (rc) return rc;
   OpDesc db;
-  if ((rc = batched_tile(d, P, pl, nrhs, st, db))) return rc;
+  if ((rc = batched_tile(d, P, pl, nrhs, P.rl_p, P.rl_t, st, db))) return rc;
   const int gwl = wilson_blocks(db, pl, nrhs), gw64 = wilson_blocks(db, P64, nrhs);
   // the update accumulation is mostly gated off: a small grid-stride grid
   const int gsr = std::min(gs, 1024);
