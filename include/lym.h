@@ -278,8 +278,10 @@ int lym_bicgstab(const lym_wilson* op, int square, double shift, const void* b, 
  *   prec = LYM_PREC_F64: FP64 throughout.
  *   prec = LYM_PREC_F32, several shifts: FP64 Krylov vectors until the base
  *          residual is below 3e-4 |b|, complex64 after; shift solutions and
- *          directions accumulate in FP64 (deferred block updates); FP64
- *          defect correction of any shift whose true residual misses tol.
+ *          directions accumulate in FP64 (deferred block updates over 64
+ *          FP64 / 128 low-precision iterations, whose residuals the workspace
+ *          keeps: about 1 KB per complex number of the field); FP64 defect
+ *          correction of any shift whose true residual misses tol.
  *   prec = LYM_PREC_F16, several shifts, fundamental representation: as
  *          LYM_PREC_F32, then 16-bit storage of the Krylov vectors (int16
  *          fixed point scaled by a per-site float norm, 4 bytes per complex
