@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define LYM_ABI_VERSION 4
+#define LYM_ABI_VERSION 5
 
 enum {
   LYM_OK = 0,
@@ -59,8 +59,8 @@ typedef struct lym_wilson {
   double bc_re, bc_im;    /* boundary factor, (1,0) = periodic */
   const void* links;            /* complex128 */
   const void* links_right;      /* complex128 or NULL (adjoint only) */
-  const void* links_f32;        /* complex64 copy of links, or NULL */
-  const void* links_right_f32;  /* complex64 copy of links_right, or NULL */
+  const void* links_f32;        /* complex64 copy of links (lym_pack_links_f32 layout), or NULL */
+  const void* links_right_f32;  /* the same of links_right, or NULL */
   /* LYM_LINKS_SU: every link is special unitary (checked by the caller with
    * lym_link_deviation); fundamental-representation kernels then read the
    * first nc-1 rows of each link and rebuild the last one (SU(3): row 2 =
@@ -146,6 +146,15 @@ int lym_spin_axpby(int ns, int64_t n_per_spin, const double* coef, const void* x
 /* Precision conversion of n complex numbers (F64 <-> F32). No reference
  * counterpart: the reference runs in complex128 only. */
 int lym_convert(int prec_in, const void* in, int prec_out, void* out, int64_t n, void* stream);
+/* complex64 copy of the complex128 links (ndim, nc, nc, volume) in the layout
+ * the kernels read (lym_wilson.links_f32): per direction mu, U = ceil(nc^2/2)
+ * planes of `volume` 16-byte units; unit q of (mu, site) holds the row-major
+ * matrix elements 2q and 2q+1 as two complex64 numbers (zero padding after
+ * the last element), so a kernel lane reads a link in 16-byte loads.
+ * lym_links_f32_bytes gives the size (0 for bad arguments).  No reference
+ * counterpart (the reference keeps complex128 links only, lattice.py). */
+size_t lym_links_f32_bytes(int32_t ndim, int32_t nc, int64_t volume);
+int lym_pack_links_f32(int32_t ndim, int32_t nc, int64_t volume, const void* links, void* links_f32, void* stream);
 /* dev_out[0] (device double) = max over links and entries of
  * |stored last row - row rebuilt from the others| for op->links (F64) or
  * op->links_f32 (F32), the test that allows LYM_LINKS_SU.  No reference

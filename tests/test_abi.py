@@ -53,3 +53,16 @@ def test_struct_layout_matches_header():
     # 4 int32 + 4 int32 dims + 3 double + 4 pointers + link_format + reserved
     assert ctypes.sizeof(_lib.LymWilson) == 4 * 4 + 4 * 4 + 3 * 8 + 4 * 8 + 2 * 4
     assert ctypes.sizeof(_lib.LymSolveStats) == 4 * 4 + 8 + 8 + 64 * 4 + 64 * 8 + 64 * 8 + 64 * 4 + 2 * 4
+
+
+def test_links_f32_bytes(lib):
+    """Packed complex64 links: ceil(nc^2 / 2) 16-byte units per direction and
+    site; unsupported shapes give 0."""
+    lib.lym_links_f32_bytes.restype = ctypes.c_size_t
+    lib.lym_links_f32_bytes.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64]
+    assert lib.lym_links_f32_bytes(4, 3, 1000) == 4 * 5 * 1000 * 16
+    assert lib.lym_links_f32_bytes(4, 2, 1000) == 4 * 2 * 1000 * 16
+    assert lib.lym_links_f32_bytes(2, 3, 384) == 2 * 5 * 384 * 16
+    assert lib.lym_links_f32_bytes(3, 3, 10) == 0
+    assert lib.lym_links_f32_bytes(4, 4, 10) == 0
+    assert lib.lym_links_f32_bytes(4, 3, 0) == 0

@@ -375,3 +375,24 @@ def test_gamma5_and_errors():
         dirac.DiracWilsonOperator(lattice.Configuration(U3, (2, 2, 2), 2, 3), 0.1)
     with pytest.raises(RuntimeError):
         op.apply(np.zeros((2, 2, 15), dtype=np.complex128))
+
+
+@pytest.mark.parametrize("ndim,nc", [(4, 3), (4, 2), (2, 3), (2, 2)])
+def test_pack_links_f32_layout(ndim, nc):
+    """lym_pack_links_f32 against the layout it documents: per direction,
+    ceil(nc^2 / 2) planes of 16-byte units; unit q of a site holds the
+    row-major elements 2q and 2q + 1 as complex64, zero after the last."""
+    from tfleonardym_amd import _lib
+    dev = _lib.require_device()
+    rng = np.random.default_rng(100 + 10 * ndim + nc)
+    V = 96
+    U = rng.standard_normal((ndim, nc, nc, V)) + 1j * rng.standard_normal((ndim, nc, nc, V))
+    packed = _lib.pack_links_f32(torch.from_numpy(U).to(dev)).cpu().numpy()
+    qu = (nc * nc + 1) // 2
+    units = packed.reshape(ndim, qu, V, 4)
+    flat = U.reshape(ndim, nc * nc, V).astype(np.complex64)
+    for q in range(qu):
+        for half, e in ((0, 2 * q), (1, 2 * q + 1)):
+            want = flat[:, e, :] if e < nc * nc else np.zeros((ndim, V), np.complex64)
+            assert np.array_equal(units[:, q, :, 2 * half], want.real)
+            assert np.array_equal(units[:, q, :, 2 * half + 1], want.imag)

@@ -15,7 +15,7 @@ PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 REP_FUNDAMENTAL, REP_ADJOINT = 0, 1
 MODE_PROJECTED, MODE_SIMPLE = 0, 1
 LINKS_FULL, LINKS_SU = 0, 1
-ABI_VERSION = 4  # LYM_ABI_VERSION of include/lym.h
+ABI_VERSION = 5  # LYM_ABI_VERSION of include/lym.h
 
 OK = 0
 NOT_CONVERGED = 1
@@ -122,6 +122,8 @@ SIGNATURES = {
     "lym_gamma5": (_i32, [_i32, _i32, _i64, _vp, _vp, _vp]),
     "lym_spin_axpby": (_i32, [_i32, _i64, ctypes.POINTER(_f64), _vp, _vp, _vp, _vp]),
     "lym_convert": (_i32, [_i32, _vp, _i32, _vp, _i64, _vp]),
+    "lym_links_f32_bytes": (_sz, [_i32, _i32, _i64]),
+    "lym_pack_links_f32": (_i32, [_i32, _i32, _i64, _vp, _vp, _vp]),
     "lym_link_deviation": (_i32, [_vp, _i32, _vp, _vp]),
     "lym_lie_derivative": (_i32, [ctypes.POINTER(LymWilson), _i32, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                   ctypes.POINTER(_f64), _i32, _i32, _vp, _vp]),
@@ -219,6 +221,22 @@ def su_link_format(desc, device=None):
     finally:
         desc.link_format = saved
     return LINKS_SU if float(out.item()) <= SU_TOLERANCE else LINKS_FULL
+
+
+def pack_links_f32(links):
+    """complex64 copy of complex128 links (ndim, nc, nc, *geometry) in the
+    kernels' packed layout (lym_pack_links_f32), as a flat float32 tensor on
+    the links' device."""
+    ndim, nc = int(links.shape[0]), int(links.shape[1])
+    volume = int(links.numel() // (ndim * nc * nc))
+    nbytes = int(load().lym_links_f32_bytes(ndim, nc, volume))
+    if nbytes == 0:
+        raise ValueError(f"pack_links_f32: unsupported links of shape {tuple(links.shape)}")
+    out = torch.empty(nbytes // 4, dtype=torch.float32, device=links.device)
+    src = links.contiguous()
+    check(load().lym_pack_links_f32(ndim, nc, volume, ptr(src), ptr(out), stream_handle(links.device)),
+          "pack links")
+    return out
 
 
 _scratch = {}

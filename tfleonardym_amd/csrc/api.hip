@@ -52,10 +52,11 @@ int check_buffers(const OpDesc* d, hipStream_t st, std::initializer_list<Buf> bu
   (void)hipGetLastError();
   if (d) {
     const size_t lb = (size_t)d->ndim * d->nc * d->nc * d->lat.V;  // complex numbers per link field
+    const size_t lb32 = (size_t)d->ndim * link_units32(d->nc) * d->lat.V * 16;  // packed complex64 links
     int rc = check_one(d->links64, lb * 16, "links");
     if (!rc) rc = check_one(d->linksr64, lb * 16, "links_right");
-    if (!rc) rc = check_one(d->links32, lb * 8, "links_f32");
-    if (!rc) rc = check_one(d->linksr32, lb * 8, "links_right_f32");
+    if (!rc) rc = check_one(d->links32, lb32, "links_f32");
+    if (!rc) rc = check_one(d->linksr32, lb32, "links_right_f32");
     if (rc) return rc;
   }
   for (const Buf& b : bufs) {
@@ -645,6 +646,24 @@ int lym_convert(int prec_in, const void* in, int prec_out, void* out, int64_t n,
   }
   if (n == 0) return LYM_OK;
   return blas_convert(prec_in, in, prec_out, out, n, (hipStream_t)stream);
+}
+
+size_t lym_links_f32_bytes(int32_t ndim, int32_t nc, int64_t volume) {
+  if ((ndim != 2 && ndim != 4) || nc < 2 || nc > 3 || volume < 1) return 0;
+  return (size_t)ndim * link_units32(nc) * (size_t)volume * 16;
+}
+
+int lym_pack_links_f32(int32_t ndim, int32_t nc, int64_t volume, const void* links, void* links_f32, void* stream) {
+  const size_t nb = lym_links_f32_bytes(ndim, nc, volume);
+  if (!links || !links_f32 || nb == 0) {
+    set_error("lym_pack_links_f32: bad arguments");
+    return LYM_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int rc = check_buffers(nullptr, st, {{links, (size_t)ndim * nc * nc * (size_t)volume * 16, "links"},
+                                             {links_f32, nb, "links_f32"}});
+  if (rc) return rc;
+  return blas_pack_links32(ndim, nc, volume, links, links_f32, st);
 }
 
 size_t lym_solver_workspace_bytes(const lym_wilson* op, int prec, int nshift) {

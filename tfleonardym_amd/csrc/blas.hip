@@ -83,6 +83,25 @@ __global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const EI* __restr
   }
 }
 
+// complex128 links (ndim, nc, nc, V) -> the kernels' packed complex64 layout
+// (ndim, U, V) of 16-byte units, U = ceil(nc^2 / 2): unit q of (mu, s) holds
+// elements 2q, 2q + 1 of the row-major matrix, zero padding after the last
+// element.  One thread per (mu, unit, site).
+__global__ __launch_bounds__(kBlock) void k_pack_links32(int nc, int64_t V, int64_t n, const double2* __restrict__ in,
+                                                         float4* __restrict__ out) {
+  const int nu = (nc * nc + 1) / 2;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int64_t s = i % V;
+    const int64_t pq = i / V;  // mu * nu + q
+    const int mu = (int)(pq / nu), q = (int)(pq % nu);
+    const int e0 = 2 * q, e1 = 2 * q + 1;
+    const double2 a = in[((int64_t)mu * nc * nc + e0) * V + s];
+    double2 b = make_double2(0.0, 0.0);
+    if (e1 < nc * nc) b = in[((int64_t)mu * nc * nc + e1) * V + s];
+    out[i] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+  }
+}
+
 template <typename E>
 __global__ __launch_bounds__(kBlock) void k_gamma5(int ns, int64_t nps, const E* __restrict__ in, E* __restrict__ out) {
   const int64_t n = (int64_t)ns * nps;
@@ -241,6 +260,14 @@ int blas_convert(int pin, const void* in, int pout, void* out, int64_t n, hipStr
     set_error("lym_convert: precisions must be F64 or F32");
     return LYM_ERR_ARG;
   }
+  LYM_HIP(hipGetLastError());
+  return LYM_OK;
+}
+
+int blas_pack_links32(int ndim, int nc, int64_t V, const void* in, void* out, hipStream_t st) {
+  const int64_t n = (int64_t)ndim * link_units32(nc) * V;
+  hipLaunchKernelGGL(k_pack_links32, dim3(stream_grid(n)), dim3(kBlock), 0, st, nc, V, n, (const double2*)in,
+                     (float4*)out);
   LYM_HIP(hipGetLastError());
   return LYM_OK;
 }

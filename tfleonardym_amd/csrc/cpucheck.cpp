@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <vector>
 
 #include "force_site.h"
 #include "lym_fin.h"
@@ -153,7 +154,18 @@ static int run_half(const Lat& lat, const float* links, const short* in, const f
   constexpr int NCOMP = Gam<D>::NS * NC;
   SpinorView<P16> sin{(short2*)in, const_cast<float*>(norm), lat.V};
   SpinorView<P32> sout{(float2*)out, nullptr, lat.V};
-  LinkView<float> lk{(const cpx<float>*)links, lat.V};
+  // the kernels read complex64 links packed in 16-byte units (lym_pack_links_f32)
+  constexpr int QU = link_units32(NC);
+  std::vector<float> packed((size_t)D * QU * lat.V * 4, 0.0f);
+  for (int mu = 0; mu < D; ++mu)
+    for (int e = 0; e < NC * NC; ++e)
+      for (int64_t s = 0; s < lat.V; ++s) {
+        const size_t dst = (((size_t)mu * QU + e / 2) * lat.V + s) * 4 + (e % 2) * 2;
+        const size_t src = (((size_t)mu * NC * NC + e) * lat.V + s) * 2;
+        packed[dst] = links[src];
+        packed[dst + 1] = links[src + 1];
+      }
+  LinkView<float> lk{(const cpx<float>*)packed.data(), lat.V};
   for (int64_t s = 0; s < lat.V; ++s) {
     cpx<float> hop[NCOMP], c[NCOMP], res[NCOMP];
     wilson_hop<D, NC, false, PROJ, float>(lat, s, sin, lk, lk, hop);

@@ -166,6 +166,7 @@ int blas_axpby(int prec, int64_t n, double ar, double ai, const void* x, double 
 int blas_dot_noalloc(int prec, int64_t n, const void* x, const void* y, double* out_dev, hipStream_t st);
 int noalloc_scratch(void** out);
 int blas_convert(int pin, const void* in, int pout, void* out, int64_t n, hipStream_t st);
+int blas_pack_links32(int ndim, int nc, int64_t V, const void* in, void* out, hipStream_t st);
 int blas_gamma5(int prec, int ns, int64_t nps, const void* in, void* out, hipStream_t st);
 // out = a x + b y with coefficients per spin half: v = {a_up, a_lo, b_up, b_lo} (re, im)
 struct SpinCoef {

@@ -337,10 +337,17 @@ struct SpinorView {
   }
 };
 
-// Links: (D, Nc, Nc, V) complex, site fastest.  Link precision is P64 or P32.
+// Links.  complex128: (D, Nc, Nc, V) complex, site fastest (the caller's
+// layout).  complex64: packed in 16-byte units, (D, U, V) with U = ceil(Nc^2 / 2)
+// units per link: unit q of (mu, s) holds elements 2q and 2q + 1 of the
+// row-major matrix (the second half of the last unit is padding when Nc^2 is
+// odd), so a lane reads a link in half the load instructions; rows 0..NC-2
+// are the first NC (NC - 1) / 2 units (NC (NC - 1) is even).
 // recon = 1: the links are special unitary; only rows 0..NC-2 are read and
 // the last row is rebuilt (SU(3): u2 = conj(u0 x u1); SU(2): u1 = (-u01*, u00*)),
 // one third (SU(3)) or half (SU(2)) fewer link bytes per hop.
+__host__ __device__ constexpr int link_units32(int nc) { return (nc * nc + 1) / 2; }
+
 template <typename R>
 struct LinkView {
   const cpx<R>* data;
@@ -348,29 +355,61 @@ struct LinkView {
   int recon = 0;
   template <int NC>
   __host__ __device__ __forceinline__ void load(int mu, int64_t s, cpx<R> (&u)[NC][NC]) const {
-    const uint32_t bytes = (uint32_t)((int64_t)(mu + 1) * NC * NC * V * (int64_t)sizeof(cpx<R>));
-#pragma unroll
-    for (int i = 0; i < NC - 1; ++i)
-#pragma unroll
-      for (int j = 0; j < NC; ++j) u[i][j] = ld<cpx<R>>(data, bytes, (int64_t)(mu * NC + i) * NC + j, V, s);
-    if (recon) {
-      if constexpr (NC == 3) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          // u[2][j] = conj(u0a u1b - u0b u1a)
-          const int a = (j + 1) % 3, b = (j + 2) % 3;
-          const cpx<R> x = u[0][a], y = u[1][b], z = u[0][b], w = u[1][a];
-          const R re = fmad(x.re, y.re, fmad(-x.im, y.im, fmad(-z.re, w.re, z.im * w.im)));
-          const R im = fmad(z.re, w.im, fmad(z.im, w.re, fmad(-x.re, y.im, -(x.im * y.re))));
-          u[2][j] = mk<R>(re, im);
-        }
-      } else if constexpr (NC == 2) {
-        u[1][0] = mk<R>(-u[0][1].re, u[0][1].im);
-        u[1][1] = mk<R>(u[0][0].re, -u[0][0].im);
-      }
+    if constexpr (sizeof(R) == 4) {
+      load_packed<NC>(mu, s, u);
     } else {
+      const uint32_t bytes = (uint32_t)((int64_t)(mu + 1) * NC * NC * V * (int64_t)sizeof(cpx<R>));
 #pragma unroll
-      for (int j = 0; j < NC; ++j) u[NC - 1][j] = ld<cpx<R>>(data, bytes, (int64_t)(mu * NC + NC - 1) * NC + j, V, s);
+      for (int i = 0; i < NC - 1; ++i)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) u[i][j] = ld<cpx<R>>(data, bytes, (int64_t)(mu * NC + i) * NC + j, V, s);
+      if (recon) {
+        rebuild<NC>(u);
+      } else {
+#pragma unroll
+        for (int j = 0; j < NC; ++j)
+          u[NC - 1][j] = ld<cpx<R>>(data, bytes, (int64_t)(mu * NC + NC - 1) * NC + j, V, s);
+      }
+    }
+  }
+  // complex64 units (q0 <= q < q1) of link (mu, s) into u
+  template <int NC>
+  __host__ __device__ __forceinline__ void load_units(int mu, int64_t s, int q0, int q1, uint32_t bytes,
+                                                      cpx<R> (&u)[NC][NC]) const {
+    constexpr int QU = link_units32(NC);
+#pragma unroll
+    for (int q = q0; q < q1; ++q) {
+      const float4 w = ld<float4>((const float4*)data, bytes, (int64_t)mu * QU + q, V, s);
+      u[(2 * q) / NC][(2 * q) % NC] = mk<R>(w.x, w.y);
+      if (2 * q + 1 < NC * NC) u[(2 * q + 1) / NC][(2 * q + 1) % NC] = mk<R>(w.z, w.w);
+    }
+  }
+  template <int NC>
+  __host__ __device__ __forceinline__ void load_packed(int mu, int64_t s, cpx<R> (&u)[NC][NC]) const {
+    constexpr int QU = link_units32(NC), QR = NC * (NC - 1) / 2;
+    const uint32_t bytes = (uint32_t)((int64_t)(mu + 1) * QU * V * 16);
+    load_units<NC>(mu, s, 0, QR, bytes, u);
+    if (recon)
+      rebuild<NC>(u);
+    else
+      load_units<NC>(mu, s, QR, QU, bytes, u);
+  }
+  // the last row of a special unitary link from rows 0..NC-2
+  template <int NC>
+  __host__ __device__ __forceinline__ static void rebuild(cpx<R> (&u)[NC][NC]) {
+    if constexpr (NC == 3) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        // u[2][j] = conj(u0a u1b - u0b u1a)
+        const int a = (j + 1) % 3, b = (j + 2) % 3;
+        const cpx<R> x = u[0][a], y = u[1][b], z = u[0][b], w = u[1][a];
+        const R re = fmad(x.re, y.re, fmad(-x.im, y.im, fmad(-z.re, w.re, z.im * w.im)));
+        const R im = fmad(z.re, w.im, fmad(z.im, w.re, fmad(-x.re, y.im, -(x.im * y.re))));
+        u[2][j] = mk<R>(re, im);
+      }
+    } else if constexpr (NC == 2) {
+      u[1][0] = mk<R>(-u[0][1].re, u[0][1].im);
+      u[1][1] = mk<R>(u[0][0].re, -u[0][0].im);
     }
   }
 };

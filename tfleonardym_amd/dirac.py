@@ -391,13 +391,10 @@ class DiracWilsonOperator(Operator):
     def ensure_f32(self):
         """complex64 copy of the links for the mixed-precision solvers."""
         if self._links32 is None:
-            self._links32 = torch.empty(self._links.shape, dtype=torch.complex64, device=self._links.device)
-            _lib.check(_lib.lib().lym_convert(_lib.PREC_F64, _lib.ptr(self._links), _lib.PREC_F32,
-                                              _lib.ptr(self._links32), self._links.numel(),
-                                              _lib.stream_handle(self._links.device)), "convert")
+            self._links32 = _lib.pack_links_f32(self._links)
             self._desc.links_f32 = self._links32.data_ptr()
             if self._links_right is not None:
-                self._links_right32 = self._links_right.to(torch.complex64)
+                self._links_right32 = _lib.pack_links_f32(self._links_right)
                 self._desc.links_right_f32 = self._links_right32.data_ptr()
         return self._desc
 

@@ -87,9 +87,7 @@ class GaugeStore:
         geometry = tuple(int(L) for L in geometry)
         if int(np.prod(geometry)) != U.shape[-1] or len(geometry) != U.shape[0]:
             raise RuntimeError("geometry does not match the gauge field")
-        U32 = torch.empty(U.shape, dtype=torch.complex64, device=self.device)
-        _lib.check(_lib.lib().lym_convert(_lib.PREC_F64, _lib.ptr(U), _lib.PREC_F32, _lib.ptr(U32), U.numel(),
-                                          _lib.stream_handle(self.device)), "convert")
+        U32 = _lib.pack_links_f32(U)
         torch.cuda.synchronize(self.device)
         d = _lib.LymWilson()
         d.ndim = U.shape[0]
@@ -120,7 +118,7 @@ class GaugeStore:
         return (ns, e["U"].shape[1], e["U"].shape[-1])
 
     def resident_bytes(self):
-        return sum(e["U"].numel() * 16 + e["U32"].numel() * 8 for e in self._entries.values())
+        return sum(e["U"].numel() * 16 + e["U32"].numel() * e["U32"].element_size() for e in self._entries.values())
 
     def keys(self):
         return list(self._entries)
