@@ -104,7 +104,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       default: run = true;
     }
   }
-  if (!run) return;
+  // the partial sums and the per-shift inputs are read before the phase test
+  // resolves, so their loads overlap the load of the phase (a gated-out
+  // launch reads them for nothing); nothing is written before the test
   double sums[4] = {0.0, 0.0, 0.0, 0.0};
   if (nr == 1) {
     double s[1];
@@ -122,6 +124,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
     sums[1] = s[1];
     sums[2] = s[2];
   }
+  __shared__ ShiftIn shifts;
+  if (kind == FK_CGM_A || kind == FK_CG_A || kind == FK_CGM_B || kind == FK_CGM_BF) stage_shifts(st, shifts);
+  if (!run) return;
   // kinds whose per-shift recurrences run on every thread of the block
   switch (kind) {
     case FK_INIT:
@@ -133,8 +138,6 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
       break;
     default: break;
   }
-  __shared__ ShiftIn shifts;
-  if (kind == FK_CGM_A || kind == FK_CG_A || kind == FK_CGM_B || kind == FK_CGM_BF) stage_shifts(st, shifts);
   if (threadIdx.x != 0) return;
   switch (kind) {
     case FK_SUM:
