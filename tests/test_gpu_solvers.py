@@ -152,6 +152,34 @@ def test_half_storage_multishift(geometry, nc, tune, monkeypatch):
             assert rel(xn, x1.cpu().numpy()) < 1e-8, sh
 
 
+@pytest.mark.parametrize("nshift,tune", [(6, "1e-2,1e-4,0.5,16,32"), (6, "1e-2,1e-4,0.5,32,64"),
+                                         (30, "1e-2,1e-4,0.5"), (13, "3e-4,1e-6,0.5")])
+def test_fast_cgm_blocks_and_groups(nshift, tune, monkeypatch):
+    """The deferred shift updates of the fast multishift CG, folded per block:
+    blocks shorter than a phase (16 / 32 and 32 / 64 iterations instead of
+    64 / 128, so every phase holds several blocks and each precision switch
+    flushes one in progress), and more shifts than one update group holds
+    (30 = 3 groups of 10, 13 = groups of 7 and 6).  16-bit storage after the
+    second threshold.  Every shift reaches 1e-10 (true residual recomputed by
+    the oracle) and matches the oracle's FP64 CG-M to 1e-8."""
+    from tfleonardym_amd import dirac, dirac_solver
+    monkeypatch.setenv("LYM_CGM_TUNE", tune)
+    geometry, nc = (8, 4, 4, 16), 3
+    print(f"case fast cgm nshift={nshift} tune={tune}", flush=True)
+    op, oop = setup(geometry, nc, 0.13)
+    b = fields.gaussian_spinor(geometry, 4, nc, seed=90 + nshift)
+    shifts = fields.log_shifts(1e-3, 5.0, nshift)
+    s = dirac_solver.multishift_solver(5000, 0.0, algorithm="cg", tolerance=1e-10, precision_mode="half",
+                                       verbose=False)
+    xs = s.solve(dirac.SquareOperator(op), b, shifts)
+    assert s.last_status == "converged", s.last_stats
+    ref, _ = osv.cgm(oop.apply_square, b, shifts, tol=1e-12)
+    for sh, x, xr in zip(shifts, xs, ref):
+        xn = x.cpu().numpy()
+        assert osv.true_residual(oop.apply_square, b, xn, sh) <= 1e-10, sh
+        assert rel(xn, xr) < 1e-8, sh
+
+
 def test_half_multishift_zero_source(monkeypatch):
     """A zero source next to a Gaussian one in a 16-bit multishift batch: the
     zero source is solved by x = 0 at once and does not hold back the
