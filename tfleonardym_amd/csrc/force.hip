@@ -165,6 +165,79 @@ __global__ __launch_bounds__(force_lds_threads<NC>()) void k_force_lds(Lat lat, 
     }
 }
 
+// Pair-parallel kernel (4D fundamental, hermitian): a block covers nsite
+// sites (nsite / T whole lines of the fastest direction, the tiled schedule)
+// x D directions x PP pair slots; slot q sums the pairs p = q, q + PP, ...
+// of its (site, direction) into P, the slots are added in LDS in a fixed
+// order (slot PP-1 first, slot 0 last), and slot 0 applies the link and
+// writes.  A block lives for about n / PP pairs instead of n, so the blocks
+// in flight on an XCD work on fewer pairs at a time and the x + mu
+// neighbours a block reads were fetched by the blocks next to it.
+template <int NC, int PP>
+__global__ __launch_bounds__(1024) void k_force_pp(Lat lat, SiteMap map, int nsite, LinkView<double> lk,
+                                                   ForcePairs fp, uint32_t vbytes, cpx<double>* out,
+                                                   uint32_t obytes) {
+  constexpr int D = 4;
+  extern __shared__ double2 force_pp_lds[];  // [D][nsite][NC * NC]
+  const int t = (int)threadIdx.x;
+  const int sl = t % nsite;
+  const int mu = (t / nsite) % D;
+  const int pq = t / (nsite * D);
+  int64_t s;
+  int x[D];
+  site_of<D>(lat, map, (int)blockIdx.x, (int)gridDim.x, sl, nsite, s, x);
+  const int L = lat.L[mu];
+  const int64_t st = lat.stride[mu];
+  const int64_t nb = (x[mu] == L - 1) ? s - (int64_t)(L - 1) * st : s + st;
+  cpx<double> P[NC][NC], Q[NC][NC];
+#pragma unroll
+  for (int i = 0; i < NC; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) P[i][j] = Q[i][j] = czero<double>();
+  for (int p = pq; p < fp.n; p += PP) {
+    force_term_fund<D, NC, true>(lat, mu, fp.x[p], fp.y[p], vbytes, s, nb, fp.w[p], P, Q);
+    if (fp.sym) force_term_fund<D, NC, true>(lat, mu, fp.y[p], fp.x[p], vbytes, s, nb, fp.w[p], P, Q);
+  }
+  double2* slot = force_pp_lds + ((int64_t)mu * nsite + sl) * NC * NC;
+  if (pq == PP - 1) {
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int j = 0; j < NC; ++j) slot[i * NC + j] = make_double2(P[i][j].re, P[i][j].im);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = PP - 2; q >= 0; --q) {
+    if (pq == q) {
+#pragma unroll
+      for (int i = 0; i < NC; ++i)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+          const double2 v = slot[i * NC + j];
+          P[i][j] = mk<double>(v.x + P[i][j].re, v.y + P[i][j].im);
+          if (q > 0) slot[i * NC + j] = make_double2(P[i][j].re, P[i][j].im);
+        }
+    }
+    __syncthreads();
+  }
+  if (pq != 0) return;
+  const bool edge = lat.has_bc && mu == D - 1 && x[mu] == L - 1;
+  cpx<double> u[NC][NC];
+  lk.template load<NC>(mu, s, u);
+  force_close_fund<NC, true>(u, edge, mk<double>(lat.bc_re, lat.bc_im), P, Q);
+  cpx<double> res[NC][NC];
+  force_finish<NC, false>(lat, P, res);
+#pragma unroll
+  for (int i = 0; i < NC; ++i)
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      const int64_t pl = ((int64_t)mu * NC + i) * NC + j;
+      cpx<double> v = res[i][j];
+      if (fp.accumulate) v += ld<cpx<double>>(out, obytes, pl, lat.V, s);
+      put<cpx<double>>(out, obytes, pl, lat.V, s, v);
+    }
+}
+
 template <int D, int NC, bool ADJ, bool HERM>
 static int launch_force(const OpDesc& d, const ForcePairs& fp, cpx<double>* out, hipStream_t st) {
   const int64_t V = d.lat.V;
@@ -184,6 +257,26 @@ static int launch_force(const OpDesc& d, const ForcePairs& fp, cpx<double>* out,
                          d.map, d.wthreads, lk, fp, vbytes, out, obytes);
       LYM_HIP(hipGetLastError());
       return LYM_OK;
+    }
+  }
+  // three or more pairs: two pair slots per (site, direction) over blocks of
+  // two x3 lines (measured at 24^3 x 48: 4 pairs 705 against 770 us, 8 pairs
+  // 1300 against 1679 us; with one or two pairs the idle slots cost more
+  // than the shorter block lifetime gains, 313 / 431 against 255 / 380 us)
+  if constexpr (D == 4 && !ADJ && HERM) {
+    constexpr int kl = 2;
+    if (fp.n >= 3 && d.map.tiled && d.lat.L[2] % kl == 0) {
+      const int T = d.map.T;
+      const int nsite = T * kl;
+      const SiteMap m{1, T, kl, d.map.slab, d.lat.L[2] / kl};
+      const size_t lds = (size_t)D * nsite * NC * NC * 16;
+      const int nthr = nsite * D * 2;
+      if (nthr <= 1024 && nthr % 64 == 0 && lds <= 64 * 1024) {
+        hipLaunchKernelGGL((k_force_pp<NC, 2>), dim3((unsigned)(d.lat.V / nsite)), dim3(nthr), lds, st, d.lat, m,
+                           nsite, lk, fp, vbytes, out, obytes);
+        LYM_HIP(hipGetLastError());
+        return LYM_OK;
+      }
     }
   }
   // (the non-hermitian SU(3) form needs more registers than 1024-thread blocks allow)
