@@ -297,7 +297,9 @@ int lym_bicgstab(const lym_wilson* op, int square, double shift, const void* b, 
  *          number; FP32 arithmetic) below 1e-6 |b|.
  *   prec = LYM_PREC_F32, one shift: complex64 CG with FP64 reliable updates
  *          (true residual recomputed in FP64 whenever the iterated one has
- *          dropped 100x, and at the end).
+ *          dropped 100x, and at the end); the solution accumulates in FP64
+ *          in blocks of 16 steps from the last 16 directions, which the
+ *          workspace keeps (16 fields per source at the storage precision).
  *   prec = LYM_PREC_F16, one shift, fundamental representation: the same
  *          with 16-bit storage of r, p and Q p; arithmetic in FP32.
  * b and xs are complex128.  Synchronous.  New entry point (no counterpart in

@@ -180,8 +180,14 @@ __device__ inline void cgm_fin_b_fast(SolverState* st, const ShiftIn& in, double
 // <r_k, r_k> so the update can form beta from the true residual).  rr_max is
 // the true residual of the last update (|b|^2 before the first).
 __device__ inline void cg_fin_b_rel(SolverState* st, double rrn) {
+  // step blk_j of the current solution block: x_hi += alpha p is deferred to
+  // the block's fold (rel_fold_due); beta is recorded below when the
+  // recurrence continues
+  const int j = st->blk_j;
+  st->h_alpha[0][j] = st->alpha;
+  st->h_beta[0][j] = 0.0;
+  st->blk_j = j + 1;
   st->iter += 1;
-  st->x_pending = 1;  // x_lo += alpha p happens with the next p update (or the reliable update)
   if (!isfinite(rrn)) {
     st->status = ST_BREAKDOWN;
     st->phase = PH_DONE;
@@ -195,10 +201,17 @@ __device__ inline void cg_fin_b_rel(SolverState* st, double rrn) {
     return;
   }
   st->beta_old = rrn / st->rr;
+  st->h_beta[0][j] = st->beta_old;
   st->kp_a_re = 1.0;
   st->kp_a_im = 0.0;
   st->rr = rrn;
   st->phase = PH_KP;
+}
+
+// The mixed CG's deferred solution update runs (and its block restarts)
+// before a reliable update, or when the block is full while iterating.
+__host__ __device__ inline bool rel_fold_due(const SolverState* st) {
+  return (st->phase == PH_RU && st->blk_j > 0) || (st->phase == PH_KP && st->blk_j >= st->mblk);
 }
 
 // After the FP64 true residual of a reliable update: converged when it meets
@@ -209,7 +222,7 @@ __device__ inline void cg_fin_b_rel(SolverState* st, double rrn) {
 // arXiv:0911.3191), which low-precision storage of p otherwise erodes.
 __device__ inline void cg_fin_ru(SolverState* st, double rr_true, double rp_re, double rp_im) {
   st->n_ru += 1;
-  st->x_pending = 0;  // folded into x_hi by the update
+  st->blk_j = 0;  // the block was folded into x_hi before the update
   if (!isfinite(rr_true)) {
     st->status = ST_BREAKDOWN;
     st->phase = PH_DONE;

@@ -48,7 +48,8 @@ struct EpiArgs {
   Field aux1;  // secondary in/out
   double sigma = 0.0;
   SolverState* st = nullptr;
-  // fast CG-M residual ring: nslot fields of `slot_elems` elements each
+  // fast CG-M residual ring / mixed CG direction ring: nslot fields of
+  // `slot_elems` elements each
   Field ring;
   int64_t slot_elems = 0;
   int nslot = 0;
@@ -105,12 +106,13 @@ enum EpiKind : int {
   EPI_MR_K1 = 4,     // out = Op in (phase gated)
   EPI_MR_K2 = 5,     // out = Op in + sigma aux0; reduce <out,aux0>, <out,out>; MR finalizer
   EPI_CGM_K2F = 6,   // fast CG-M: ring[k+1] = ring[k] - alpha (Op in + sigma0 aux0); deferred finalizer
-  EPI_CG_K2X = 7,    // mixed CG: aux1 -= alpha (Op in + sigma0 aux0); reduce |aux1|^2
+  EPI_CG_K2X = 7,    // mixed CG: aux1 -= alpha (Op in + sigma0 p_k from the ring); reduce |aux1|^2
   EPI_RU1 = 8,       // reliable update (complex128, gate RU): out = Op in
-  EPI_RU2 = 9,       // reliable update: r = bs - (Op in + sigma0 aux0) -> out (complex64); reduce |r|^2
+  EPI_RU2 = 9,       // reliable update: r = bs - (Op in + sigma0 aux0) -> out (complex64); reduce |r|^2, <r, p>
   EPI_RU2H = 10,     // the same with out in half storage (per-site norms)
   EPI_GSTORE = 11,   // out = Op in, only while st->phase == PH_K1 (BiCGStab)
   EPI_GAXPY = 12,    // out = Op in + sigma aux0, only while st->phase == PH_K1
+  EPI_CG_K1R = 13,   // mixed CG: out = Op p_k, input = ring slot iter of each RHS; reduce |out|^2, |p|^2
 };
 int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, const EpiArgs& a, RedCtl rc,
                   hipStream_t stream, int nrhs = 1);

@@ -80,7 +80,6 @@ struct SolverState {
   double kp_a_re, kp_a_im;
   int ru_final;   // the pending update follows the iterated stopping test
   int n_ru;       // reliable updates done
-  int x_pending;  // x_lo += alpha p of the last step is still to be applied
   // ---- BiCGStab (dirac_solver.py:56-110): complex scalars (re, im)
   double bi_rho[2], bi_rho_next[2], bi_alpha[2], bi_omega[2], bi_beta[2];
   double bi_shift;  // A = Op (+ Op) + shift

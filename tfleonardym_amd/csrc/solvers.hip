@@ -201,7 +201,10 @@ __global__ __launch_bounds__(kBlock) void k_finalize(SolverState* st, const doub
     case FK_CG_A:
     case FK_CG_B:
     case FK_CG_RU:
-      if (kind == FK_CG_A) cgm_fin_a(st, shifts, sums[0], sums[1]);
+      if (kind == FK_CG_A) {
+        if (st->blk_j >= st->mblk) st->blk_j = 0;  // k_rel_kp folded the full block
+        cgm_fin_a(st, shifts, sums[0], sums[1]);
+      }
       else if (kind == FK_CG_B) cg_fin_b_rel(st, sums[0]);
       else if (kind == FK_CG_RU) cg_fin_ru(st, sums[0], sums[1], sums[2]);
       if (ctl && st->phase == PH_DONE) atomicAdd(&ctl->ndone, 1);
@@ -582,9 +585,9 @@ __global__ __launch_bounds__(kBlock) void k_fast_switch16(SolverState* st, Batch
 
 // ---- mixed CG with reliable updates: site kernels (grid.y = RHS) -----------
 // A thread owns whole sites.  r, p and t = Q p are stored at precision PV
-// (complex64, or 16-bit fixed point with per-site norms); the solution since
-// the last update, x_lo, is always complex64 (16-bit x_lo under complex64
-// r, p was measured: point sources on 24^3x48 then need 2.8x the iterations).
+// (complex64, or 16-bit fixed point with per-site norms); the solution is
+// accumulated in complex128 (a 16-bit running solution under complex64 r, p
+// was measured: point sources on 24^3x48 then need 2.8x the iterations).
 // Fields of RHS i start at data + i * NCOMP * V (and norm + i * V).
 #define LYM_SITE_STRIDE(s, V) \
   for (int64_t s = (int64_t)blockIdx.x * kBlock + threadIdx.x; s < (V); s += (int64_t)gridDim.x * kBlock)
@@ -595,14 +598,27 @@ __device__ __forceinline__ SpinorView<PV> vec_view(const Field& f, int64_t V, in
                         f.norm ? f.norm + (int64_t)rhs * V : nullptr, V};
 }
 
-// init: r = b, p = 0, x_lo = 0, x_hi = 0; partial |b|^2 (FP64)
+// Steps per deferred solution block of the mixed CG, and the slots of its
+// direction ring: p_k lives in slot k % kRelBlock.  A full block's fold reads
+// p_{k0}..p_{k0+15} at a site before the direction update of step k0+16
+// overwrites slot k0 % 16 there (same thread), so the ring needs no spare slot.
+constexpr int kRelBlock = 16;
+static_assert(kRelBlock <= kMaxBlock && kRelBlock <= kBlock, "mixed CG block exceeds the recorded steps");
+
+// slot q of RHS rhs in the mixed CG's direction ring (nslot slots per RHS)
 template <int PV, int NCOMP>
-__global__ __launch_bounds__(kBlock) void k_rel_init(int64_t V, const Field* bs, Field r, Field p, Field x,
+__device__ __forceinline__ SpinorView<PV> ring_view(const Field& f, int64_t V, int rhs, int nslot, int q) {
+  const int64_t k = (int64_t)rhs * nslot + q;
+  return SpinorView<PV>{(typename Store<PV>::elem*)f.data + k * NCOMP * V, f.norm ? f.norm + k * V : nullptr, V};
+}
+
+// init: r = b, p_{-1} = 0 (ring slot nslot - 1), x_hi = 0; partial |b|^2 (FP64)
+template <int PV, int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_rel_init(int64_t V, const Field* bs, Field r, Field ring, int nslot,
                                                      double2* __restrict__ x64, double* partials) {
   const int rhs = blockIdx.y;
   const SpinorView<P64> b{(double2*)bs[rhs].data, nullptr, V};
-  const SpinorView<PV> rv = vec_view<PV, NCOMP>(r, V, rhs), pv = vec_view<PV, NCOMP>(p, V, rhs);
-  const SpinorView<P32> xv = vec_view<P32, NCOMP>(x, V, rhs);
+  const SpinorView<PV> rv = vec_view<PV, NCOMP>(r, V, rhs), pv = ring_view<PV, NCOMP>(ring, V, rhs, nslot, nslot - 1);
   const SpinorView<P64> xh{x64 + (int64_t)rhs * NCOMP * V, nullptr, V};
   partials += (int64_t)rhs * gridDim.x;
   double red[1] = {0.0};
@@ -620,85 +636,92 @@ __global__ __launch_bounds__(kBlock) void k_rel_init(int64_t V, const Field* bs,
     }
     rv.template store<NCOMP>(s, lo);
     pv.template store<NCOMP>(s, z);
-    xv.template store<NCOMP>(s, z);
     xh.template store<NCOMP>(s, z64);
   }
   block_partials<1>(red, partials);
 }
 
-// direction update (gate KP): x_lo += alpha p (the last step's, if pending),
-// then p = kp_a r + beta p
+// Deferred solution update of the mixed CG: a block of nb steps k0..k0+nb-1
+// (k0 = iter - nb) adds sum_j alpha_j p_{k0+j} to x_hi, with the stored
+// directions of the ring (the ones the iteration used), accumulated in FP64.
+__device__ inline void rel_fold_alphas(const SolverState* st, int nb, double* sa) {
+  if ((int)threadIdx.x < nb) sa[threadIdx.x] = st->h_alpha[0][threadIdx.x];
+  __syncthreads();
+}
+
 template <int PV, int NCOMP>
-__global__ __launch_bounds__(kBlock) void k_rel_kp(const SolverState* st, int64_t V, Field r, Field p, Field x) {
+__device__ __forceinline__ void rel_fold_site(int64_t s, int64_t V, const Field& ring, int nslot, int rhs, int k0,
+                                              int nb, const double* sa, const SpinorView<P64>& xh) {
+  cpx<double> w[NCOMP];
+  cpx<float> v[NCOMP];
+  xh.template load<NCOMP>(s, w);
+  for (int j = 0; j < nb; ++j) {
+    ring_view<PV, NCOMP>(ring, V, rhs, nslot, (k0 + j) % nslot).template load<NCOMP>(s, v);
+    const double a = sa[j];
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c)
+      w[c] = mk<double>(fma(a, (double)v[c].re, w[c].re), fma(a, (double)v[c].im, w[c].im));
+  }
+  xh.template store<NCOMP>(s, w);
+}
+
+// direction update (gate KP): p_k = kp_a r + beta p_{k-1} into ring slot
+// k % nslot (k = iter).  When the block is full (rel_fold_due) its solution
+// update is applied first and a new block starts at step k (k_finalize's
+// FK_CG_A clears the block counter).
+template <int PV, int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_rel_kp(const SolverState* st, int64_t V, Field r, Field ring, int nslot,
+                                                   double2* __restrict__ x64) {
   const int rhs = blockIdx.y;
   st += rhs;
   if (st->phase != PH_KP) return;
-  const SpinorView<PV> rv = vec_view<PV, NCOMP>(r, V, rhs), pv = vec_view<PV, NCOMP>(p, V, rhs);
-  const SpinorView<P32> xv = vec_view<P32, NCOMP>(x, V, rhs);
-  const float b = (float)st->beta_old, al = (float)st->alpha;
+  const int nb = st->blk_j, k = st->iter;
+  const bool fold = rel_fold_due(st);
+  __shared__ double sa[kRelBlock];
+  if (fold) rel_fold_alphas(st, nb, sa);
+  const SpinorView<PV> rv = vec_view<PV, NCOMP>(r, V, rhs),
+                       po = ring_view<PV, NCOMP>(ring, V, rhs, nslot, (k + nslot - 1) % nslot),
+                       pn = ring_view<PV, NCOMP>(ring, V, rhs, nslot, k % nslot);
+  const SpinorView<P64> xh{x64 + (int64_t)rhs * NCOMP * V, nullptr, V};
+  const float b = (float)st->beta_old;
   const float ar = (float)st->kp_a_re, ai = (float)st->kp_a_im;
-  const bool pend = st->x_pending != 0;
   LYM_SITE_STRIDE(s, V) {
+    if (fold) rel_fold_site<PV, NCOMP>(s, V, ring, nslot, rhs, k - nb, nb, sa, xh);
     cpx<float> a[NCOMP], q[NCOMP];
-    pv.template load<NCOMP>(s, q);
-    if (pend) {
-      xv.template load<NCOMP>(s, a);
-#pragma unroll
-      for (int c = 0; c < NCOMP; ++c) a[c] = mk<float>(a[c].re + al * q[c].re, a[c].im + al * q[c].im);
-      xv.template store<NCOMP>(s, a);
-    }
+    po.template load<NCOMP>(s, q);
     rv.template load<NCOMP>(s, a);
 #pragma unroll
     for (int c = 0; c < NCOMP; ++c)
       q[c] = mk<float>(ar * a[c].re - ai * a[c].im + b * q[c].re, ar * a[c].im + ai * a[c].re + b * q[c].im);
-    pv.template store<NCOMP>(s, q);
+    pn.template store<NCOMP>(s, q);
   }
 }
 
-// reliable update, accumulation (gate RU): x_hi += x_lo (+ alpha p if the
-// last step is pending); x_lo = 0
+// reliable update, accumulation (gate RU, rel_fold_due): the solution update
+// of the steps since the block started goes into x_hi (k_finalize's FK_CG_RU
+// clears the block counter)
 template <int PV, int NCOMP>
-__global__ __launch_bounds__(kBlock) void k_ru_add(const SolverState* st, int64_t V, Field x, Field p,
-                                                   double2* __restrict__ x64) {
+__global__ __launch_bounds__(kBlock) void k_ru_fold(const SolverState* st, int64_t V, Field ring, int nslot,
+                                                    double2* __restrict__ x64) {
   const int rhs = blockIdx.y;
   st += rhs;
-  if (st->phase != PH_RU) return;
-  const SpinorView<P32> xv = vec_view<P32, NCOMP>(x, V, rhs);
-  const SpinorView<PV> pv = vec_view<PV, NCOMP>(p, V, rhs);
+  if (st->phase != PH_RU || !rel_fold_due(st)) return;
+  const int nb = st->blk_j, k = st->iter;
+  __shared__ double sa[kRelBlock];
+  rel_fold_alphas(st, nb, sa);
   const SpinorView<P64> xh{x64 + (int64_t)rhs * NCOMP * V, nullptr, V};
-  const double al = st->x_pending ? st->alpha : 0.0;
-  LYM_SITE_STRIDE(s, V) {
-    cpx<float> a[NCOMP], q[NCOMP], z[NCOMP];
-    cpx<double> w[NCOMP];
-    xv.template load<NCOMP>(s, a);
-    pv.template load<NCOMP>(s, q);
-    xh.template load<NCOMP>(s, w);
-#pragma unroll
-    for (int c = 0; c < NCOMP; ++c) {
-      w[c] = mk<double>(w[c].re + (double)a[c].re + al * (double)q[c].re,
-                        w[c].im + (double)a[c].im + al * (double)q[c].im);
-      z[c] = czero<float>();
-    }
-    xh.template store<NCOMP>(s, w);
-    xv.template store<NCOMP>(s, z);
-  }
+  LYM_SITE_STRIDE(s, V) rel_fold_site<PV, NCOMP>(s, V, ring, nslot, rhs, k - nb, nb, sa, xh);
 }
 
-// result: xs[rhs] = x_hi + x_lo (x_lo is zero after a final update)
-template <int PV, int NCOMP>
-__global__ __launch_bounds__(kBlock) void k_rel_out(int64_t V, Field x, const double2* __restrict__ x64,
-                                                    const Field* xs) {
+// result: xs[rhs] = x_hi
+template <int NCOMP>
+__global__ __launch_bounds__(kBlock) void k_rel_out(int64_t V, const double2* __restrict__ x64, const Field* xs) {
   const int rhs = blockIdx.y;
-  const SpinorView<P32> xv = vec_view<P32, NCOMP>(x, V, rhs);
   const SpinorView<P64> xh{const_cast<double2*>(x64) + (int64_t)rhs * NCOMP * V, nullptr, V};
   const SpinorView<P64> out{(double2*)xs[rhs].data, nullptr, V};
   LYM_SITE_STRIDE(s, V) {
-    cpx<float> a[NCOMP];
     cpx<double> w[NCOMP];
-    xv.template load<NCOMP>(s, a);
     xh.template load<NCOMP>(s, w);
-#pragma unroll
-    for (int c = 0; c < NCOMP; ++c) w[c] = mk<double>(w[c].re + (double)a[c].re, w[c].im + (double)a[c].im);
     out.template store<NCOMP>(s, w);
   }
 }
@@ -977,10 +1000,11 @@ struct Parts {
   Field tb64, tb32, tb16;           // t = Q p
   Field ps64;               // nrhs x nshift FP64 shift directions
   // mixed CG with reliable updates (one shift), every field nrhs x batched:
-  // residual, direction, Q p (complex64 or 16-bit storage), solution since
-  // the last update (complex64)
-  Field rl_r, rl_p, rl_t, rl_x;
-  Field rl_x64, rl_t64;          // complex128 accumulated solution, Q x
+  // residual, direction ring (rl_nslot slots), Q p (complex64 or 16-bit
+  // storage)
+  Field rl_r, rl_ring, rl_t;
+  int rl_nslot = 0;
+  Field rl_x64, rl_t64;  // complex128 accumulated solution, Q x
 };
 
 // prec: P64 = plain FP64 solves (and MR); otherwise the fast mixed solver
@@ -1012,21 +1036,22 @@ static size_t carve(const OpDesc& d, int prec, int nshift, int nrhs, char* base,
       // one shift: CG with reliable updates, Krylov vectors in complex64 or
       // half storage (per-site norms, nrhs x V floats)
       const int pl = prec == P16 ? P16 : P32;
-      auto take_batch = [&](int q) {
+      auto take_batch = [&](int q, int nf) {
         Field f;
-        f.data = c.take(field_bytes(d, q) * nrhs);
-        if (q == P16) f.norm = (float*)c.take((size_t)d.lat.V * sizeof(float) * nrhs);
+        f.data = c.take(field_bytes(d, q) * nf * nrhs);
+        if (q == P16) f.norm = (float*)c.take((size_t)d.lat.V * sizeof(float) * nf * nrhs);
         return f;
       };
-      P.rl_r = take_batch(pl);
-      P.rl_p = take_batch(pl);
-      P.rl_t = take_batch(pl);
-      // x_lo stays complex64 under 16-bit r, p, t: a 16-bit x_lo perturbs
-      // the solution by ~3e-5 relative per site, which the next reliable
-      // update sees as a residual of ~3e-5 cond(A) and the solve loses ground
-      P.rl_x = take_batch(P32);
-      P.rl_x64 = take_batch(P64);
-      P.rl_t64 = take_batch(P64);
+      P.rl_nslot = kRelBlock;
+      P.rl_r = take_batch(pl, 1);
+      P.rl_ring = take_batch(pl, P.rl_nslot);
+      P.rl_t = take_batch(pl, 1);
+      // the solution is accumulated in complex128 only (each block's update
+      // is folded into it); a 16-bit running solution would perturb it by
+      // ~3e-5 relative per site, which the next reliable update sees as a
+      // residual of ~3e-5 cond(A)
+      P.rl_x64 = take_batch(P64, 1);
+      P.rl_t64 = take_batch(P64, 1);
     } else {
       const CgmTuning tun = cgm_tuning();
       P.nslot64 = tun.mblk_64 + 1;
@@ -1755,14 +1780,22 @@ static int cgm_fast(const OpDesc& d, const Parts& P, const void* const* bs, void
 // Krylov vectors and FP64 reliable updates (Sleijpen & van der Vorst; the
 // scheme of Clark et al., arXiv:0911.3191): the low-precision recurrence runs
 // until its residual has dropped by 100x since the last update (or passes the
-// stopping test), then x_hi += x_lo and r = b - (Q^2 + sigma) x_hi are formed
-// in FP64 and replace the iterated residual, p is kept.  Every RHS advances
+// stopping test), then the solution update since the last one is added to
+// x_hi and r = b - (Q^2 + sigma) x_hi is formed in FP64 and replace the iterated residual, p is kept.  Every RHS advances
 // in the same launches; one that has stopped skips them.  xs[r] (FP64) get
 // the solutions.  Iteration (each step one launch, RHS batched):
-//   x_lo += alpha p, p = r + beta p ; t = Q p (|t|^2, |p|^2) ; alpha ;
-//   r -= alpha (Q t + sigma p) (|r|^2) ; beta / update test ;
+//   p_k = r + beta p_{k-1} (ring slot k) ; t = Q p_k (|t|^2, |p|^2) ; alpha ;
+//   r -= alpha (Q t + sigma p_k) (|r|^2) ; beta / update test ;
 //   after every second iteration, for the RHS that asked for it:
-//   [x_hi += x_lo + alpha p ; t = Q x_hi ; r = b - (Q t + sigma x_hi) (|r|^2, <r,p>) ; test]
+//   [x_hi += sum alpha_j p_j ; t = Q x_hi ; r = b - (Q t + sigma x_hi) (|r|^2, <r,p>) ; test]
+// The solution update sum_j alpha_j p_j over a block of up to kRelBlock steps
+// is deferred and formed from the ring of stored directions (k_rel_kp,
+// k_ru_fold), which saves the per-step read and write of a running solution.
+// (Rebuilding the block's directions from its residuals instead, as the
+// multishift solver does, was measured here: with 16-bit p the rebuilt and
+// the stored directions drift apart by the rounding of each store, every
+// reliable update pays for the gap, and the 28-source batch at 24^3x48 took
+// 1933 iterations instead of 729.)
 constexpr int kRelChunk = 8;  // iterations per host chunk
 
 // Tile of the batched Wilson kernels of the reliable-update CG.  The shape
@@ -1901,6 +1934,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   // and 5-8% less time; the solve still ends on the FP64 true residual.
   h.delta2 = 1e-4;
   h.tol2_true = tol * tol;
+  h.mblk = kRelBlock;
   std::vector<SolverState> hs(nrhs, h);
   std::vector<Field> hb(nrhs), hx(nrhs);
   for (int r = 0; r < nrhs; ++r) {
@@ -1916,45 +1950,53 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   const int gr = red_grid_flat(n);
   int rc = dispatch([&](auto pv, auto nc) {
     hipLaunchKernelGGL((k_rel_init<decltype(pv)::value, decltype(nc)::value>), dim3(gr, nrhs), dim3(kBlock), 0, st,
-                       V, P.bs_dev, P.rl_r, P.rl_p, P.rl_x, (double2*)P.rl_x64.data, P.partials);
+                       V, P.bs_dev, P.rl_r, P.rl_ring, P.rl_nslot, (double2*)P.rl_x64.data, P.partials);
   });
   if (rc) return rc;
   rc = launch_fin(FK_INIT, P.st, P.partials, gr, 1, nullptr, st, nrhs, P.ctl);
   if (rc) return rc;
   OpDesc db;
-  if ((rc = batched_tile(d, P, pl, nrhs, P.rl_p, P.rl_t, st, db))) return rc;
+  // the tile is timed on r (set to b), a plain batched field
+  if ((rc = batched_tile(d, P, pl, nrhs, P.rl_r, P.rl_t, st, db))) return rc;
   const int gwl = wilson_blocks(db, pl, nrhs), gw64 = wilson_blocks(db, P64, nrhs);
   // the update accumulation is mostly gated off: a small grid-stride grid
   const int gsr = std::min(gs, 1024);
   static_assert(kRelChunk % 2 == 0, "the last iteration of a chunk must carry the update kernels");
-  // kernel timing: class suffix and the direction update's bytes (r and p
-  // read, p written at the storage precision; x_lo read and written in
-  // complex64)
+  // kernel timing: class suffix and the direction update's bytes (r and
+  // p_{k-1} read, p_k written at the storage precision; the block fold that
+  // one launch in kRelBlock carries per RHS is not counted)
   const std::string sfx = pname(pl);
-  const double kp_bytes = (double)V * nrhs * (3.0 * vec_bytes_site(d, pl) + 2.0 * vec_bytes_site(d, P32));
+  const double kp_bytes = (double)V * nrhs * 3.0 * vec_bytes_site(d, pl);
+  const int64_t ring_stride = n * P.rl_nslot;
   auto chunk = [&](hipStream_t st) -> int {
     for (int j = 0; j < kRelChunk; ++j) {
-      // x_lo += alpha p (last step), p = r + beta p
+      // p_k = r + beta p_{k-1} (and the fold of a full block)
       int e = kt_launch(st, ("cg_direction_update_" + sfx).c_str(), kp_bytes, -1, [&]() {
         return dispatch([&](auto pv, auto nc) {
           hipLaunchKernelGGL((k_rel_kp<decltype(pv)::value, decltype(nc)::value>), dim3(gs, nrhs), dim3(kBlock), 0,
-                             st, P.st, V, P.rl_r, P.rl_p, P.rl_x);
+                             st, P.st, V, P.rl_r, P.rl_ring, P.rl_nslot, (double2*)P.rl_x64.data);
         });
       });
       if (e) return e;
       EpiArgs a1;
       a1.out = P.rl_t;
+      a1.slot_elems = n;
+      a1.nslot = P.rl_nslot;
+      a1.ring_rhs_stride = ring_stride;
       a1.st = P.st;
       a1.rhs_stride = n;
       e = kt_launch(st, ("wilson_cg_k1_" + sfx).c_str(), wilson_bytes(d, pl, 2, nrhs), -1,
-                    [&]() { return wilson_launch(EPI_CGM_K1, db, pl, true, P.rl_p, a1, redctl(P), st, nrhs); });
+                    [&]() { return wilson_launch(EPI_CG_K1R, db, pl, true, P.rl_ring, a1, redctl(P), st, nrhs); });
       if (!e)
         e = kt_launch(st, "finalize", 0.0, -1,
                       [&]() { return launch_fin(FK_CG_A, P.st, P.partials, gwl, 2, nullptr, st, nrhs, P.ctl); });
       if (e) return e;
       EpiArgs a2;
-      a2.aux0 = P.rl_p;
       a2.aux1 = P.rl_r;
+      a2.ring = P.rl_ring;
+      a2.slot_elems = n;
+      a2.nslot = P.rl_nslot;
+      a2.ring_rhs_stride = ring_stride;
       a2.st = P.st;
       a2.rhs_stride = n;
       e = kt_launch(st, ("wilson_cg_k2x_" + sfx).c_str(), wilson_bytes(d, pl, sigma != 0.0 ? 4 : 3, nrhs), -1,
@@ -1970,8 +2012,8 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
       // gated: only the RHS in PH_RU do work, so no bytes are attributed
       e = kt_launch(st, "reliable_update", 0.0, -1, [&]() {
         int e2 = dispatch([&](auto pv, auto nc) {
-          hipLaunchKernelGGL((k_ru_add<decltype(pv)::value, decltype(nc)::value>), dim3(gsr, nrhs), dim3(kBlock), 0,
-                             st, P.st, V, P.rl_x, P.rl_p, (double2*)P.rl_x64.data);
+          hipLaunchKernelGGL((k_ru_fold<decltype(pv)::value, decltype(nc)::value>), dim3(gsr, nrhs), dim3(kBlock),
+                             0, st, P.st, V, P.rl_ring, P.rl_nslot, (double2*)P.rl_x64.data);
         });
         if (e2) return e2;
         EpiArgs a3;
@@ -1982,8 +2024,11 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
         if (e2) return e2;
         EpiArgs a4;
         a4.out = P.rl_r;
+        a4.ring = P.rl_ring;
+        a4.slot_elems = n;
+        a4.nslot = P.rl_nslot;
+        a4.ring_rhs_stride = ring_stride;
         a4.aux0 = P.rl_x64;
-        a4.aux1 = P.rl_p;
         a4.bs = P.bs_dev;
         a4.st = P.st;
         a4.rhs_stride = n;
@@ -2001,7 +2046,7 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
   const int64_t max_chunks = 2 * ((int64_t)max_iter + kRelChunk) / kRelChunk + 8;
   int64_t issued = 0, polled = 0;
   KeyBuf kb;
-  kb.add('R').add(pl).add(nrhs).add(P.st).add(P.xs_dev).add(P.rl_r.data);
+  kb.add('R').add(pl).add(nrhs).add(P.st).add(P.xs_dev).add(P.rl_ring.data);
   const std::string key = op_key(db) + kb.s;
   auto issue = [&](int slot) -> int {
     const int e = launch_chunk(key, st, chunk);
@@ -2024,9 +2069,8 @@ static int cg_reliable(const OpDesc& d, const Parts& P, int pl, const void* cons
     }
     if ((rc = issue(slot))) return rc;
   }
-  rc = dispatch([&](auto pv, auto nc) {
-    hipLaunchKernelGGL((k_rel_out<decltype(pv)::value, decltype(nc)::value>), dim3(gs, nrhs), dim3(kBlock), 0, st, V,
-                       P.rl_x,
+  rc = dispatch([&](auto, auto nc) {
+    hipLaunchKernelGGL((k_rel_out<decltype(nc)::value>), dim3(gs, nrhs), dim3(kBlock), 0, st, V,
                        (const double2*)P.rl_x64.data, P.xs_dev);
   });
   if (rc) return rc;

@@ -37,6 +37,17 @@ __device__ __forceinline__ SpinorView<P> rhs_view(SpinorView<P> v, int rhs, int6
   return v;
 }
 
+// Input view of RHS `rhs`: rhs_view, unless the epilogue picks the input
+// itself (in_view: a slot of a per-RHS ring chosen by that RHS's state)
+template <class Epi, int P>
+__device__ __forceinline__ SpinorView<P> epi_in_view(const Epi& e, const SpinorView<P>& in, int rhs,
+                                                     int64_t stride) {
+  if constexpr (requires { e.in_view(in, rhs); })
+    return e.in_view(in, rhs);
+  else
+    return rhs_view(in, rhs, stride);
+}
+
 // ---------------------------------------------------------------------------
 // Epilogues.  at(rhs) returns the epilogue of one RHS; apply() runs per site
 // after the hop is computed and adds into red[] (FP64 block partials).  `in`
@@ -290,21 +301,90 @@ struct EpiMrK2 {
   }
 };
 
-// Mixed CG step 2 (low precision): Ap = Q t + s0 p; r <- r - alpha Ap;
-// reduce |r|^2.  p is read only for a non-zero shift (x_lo += alpha p is done
-// by the next direction update, which reads p anyway).
+// The direction ring of the mixed CG: p_k (the direction of step k) of RHS i
+// in slot k % nslot at ring + i * ring_stride + (k % nslot) * slot_elems,
+// 16-bit norms at norm + (i * nslot + k % nslot) * V.  Keeping a block of
+// directions lets the solution update sum_k alpha_k p_k be deferred and
+// formed from the very directions the iteration used.
+template <int P>
+struct RingRef {
+  typename Store<P>::elem* ring;
+  float* norm;
+  int64_t slot_elems, ring_stride;
+  int nslot;
+  RingRef(const EpiArgs& a)
+      : ring((typename Store<P>::elem*)a.ring.data),
+        norm(a.ring.norm),
+        slot_elems(a.slot_elems),
+        ring_stride(a.ring_rhs_stride),
+        nslot(a.nslot) {}
+  __device__ RingRef at(int rhs, int64_t V) const {
+    RingRef r = *this;
+    r.ring = ring + rhs * ring_stride;
+    if (norm) r.norm = norm + (int64_t)rhs * nslot * V;
+    return r;
+  }
+  __device__ SpinorView<P> slot(int k, int64_t V) const {
+    const int q = k % nslot;
+    return SpinorView<P>{ring + q * slot_elems, norm ? norm + (int64_t)q * V : nullptr, V};
+  }
+};
+
+// Mixed CG step 1 (low precision, gate KP): t = Q p_k with p_k the ring slot
+// of this RHS's iteration k (the launch's input field is the ring); reduce
+// |t|^2, |p|^2.
+template <int P, int NCOMP>
+struct EpiCgK1R {
+  using R = typename Store<P>::R;
+  static constexpr int NRED = 2;
+  SpinorView<P> out;
+  SolverState* st;
+  int64_t stride, slot_elems, ring_stride;
+  int nslot;
+  EpiCgK1R(const EpiArgs& a, int64_t V)
+      : out(field_view<P>(a.out, V)), st(a.st), stride(a.rhs_stride), slot_elems(a.slot_elems),
+        ring_stride(a.ring_rhs_stride), nslot(a.nslot) {}
+  __device__ EpiCgK1R at(int rhs) const {
+    EpiCgK1R e = *this;
+    e.out = rhs_view(out, rhs, stride);
+    e.st = st + rhs;
+    return e;
+  }
+  __device__ bool active() const { return st->phase == PH_KP; }
+  __device__ SpinorView<P> in_view(const SpinorView<P>& in, int rhs) const {
+    const int q = st->iter % nslot;
+    SpinorView<P> v = in;
+    v.data += rhs * ring_stride + q * slot_elems;
+    if (v.norm) v.norm += ((int64_t)rhs * nslot + q) * in.V;
+    return v;
+  }
+  template <class CV>
+  __device__ void apply(const Lat& lat, int64_t s, const CV& in, const cpx<R> (&hop)[NCOMP],
+                        double* red) const {
+    cpx<R> c[NCOMP], res[NCOMP];
+    in.template load<NCOMP>(s, c);
+    wilson_finish<NCOMP, R>(lat, c, hop, res);
+    out.template store<NCOMP>(s, res);
+    red[0] += norm2<NCOMP, R>(res);
+    red[1] += norm2<NCOMP, R>(c);
+  }
+};
+
+// Mixed CG step 2 (low precision): Ap = Q t + s0 p_k; r <- r - alpha Ap;
+// reduce |r|^2.  p_k (ring slot k) is read only for a non-zero shift.
 template <int P, int NCOMP>
 struct EpiCgK2X {
   using R = typename Store<P>::R;
   static constexpr int NRED = 1;
-  SpinorView<P> p, r;
+  RingRef<P> pr;
+  SpinorView<P> r;
   SolverState* st;
   int64_t stride;
   EpiCgK2X(const EpiArgs& a, int64_t V)
-      : p(field_view<P>(a.aux0, V)), r(field_view<P>(a.aux1, V)), st(a.st), stride(a.rhs_stride) {}
+      : pr(a), r(field_view<P>(a.aux1, V)), st(a.st), stride(a.rhs_stride) {}
   __device__ EpiCgK2X at(int rhs) const {
     EpiCgK2X e = *this;
-    e.p = rhs_view(p, rhs, stride);
+    e.pr = pr.at(rhs, r.V);
     e.r = rhs_view(r, rhs, stride);
     e.st = st + rhs;
     return e;
@@ -320,7 +400,7 @@ struct EpiCgK2X {
     const R al = (R)st->alpha;
     if (s0 != R(0)) {
       cpx<R> pv[NCOMP];
-      p.template load<NCOMP>(s, pv);
+      pr.slot(st->iter, lat.V).template load<NCOMP>(s, pv);
 #pragma unroll
       for (int k = 0; k < NCOMP; ++k) res[k] = mk<R>(res[k].re + s0 * pv[k].re, res[k].im + s0 * pv[k].im);
     }
@@ -360,25 +440,27 @@ struct EpiRu1 {
 
 // Reliable update, second half (complex128): r = b - (Q t + s0 x_hi), the
 // true residual; reduce |r|^2 in FP64 and store r at the low precision PL
-// (complex64 or half + per-site norm) as the new iterated residual.  b of
-// RHS i is bs[i] (caller's arrays).
+// (complex64 or half + per-site norm) as the new iterated residual; reduce
+// <r, p> with the last direction (ring slot iter - 1).  b of RHS i is bs[i]
+// (caller's arrays).
 template <int P, int NCOMP, int PL>
 struct EpiRu2Base {
   using R = typename Store<P>::R;
   static constexpr int NRED = 3;  // |r|^2, <r, p> (re, im)
   SpinorView<P> x, b;
-  SpinorView<PL> rlo, p;
+  SpinorView<PL> rlo;
+  RingRef<PL> pr;
   const Field* bs;
   SolverState* st;
   int64_t stride;
   EpiRu2Base(const EpiArgs& a, int64_t V)
-      : x(field_view<P>(a.aux0, V)), b(field_view<P>(Field{}, V)), rlo(field_view<PL>(a.out, V)),
-        p(field_view<PL>(a.aux1, V)), bs(a.bs), st(a.st), stride(a.rhs_stride) {}
+      : x(field_view<P>(a.aux0, V)), b(field_view<P>(Field{}, V)), rlo(field_view<PL>(a.out, V)), pr(a),
+        bs(a.bs), st(a.st), stride(a.rhs_stride) {}
   __device__ EpiRu2Base at(int rhs) const {
     EpiRu2Base e = *this;
     e.x = rhs_view(x, rhs, stride);
     e.rlo = rhs_view(rlo, rhs, stride);
-    e.p = rhs_view(p, rhs, stride);
+    e.pr = pr.at(rhs, x.V);
     e.b.data = (typename Store<P>::elem*)bs[rhs].data;
     e.st = st + rhs;
     return e;
@@ -394,7 +476,7 @@ struct EpiRu2Base {
     const R s0 = (R)st->sigma0;
     b.template load<NCOMP>(s, c);
     cpx<float> lo[NCOMP], pv[NCOMP];
-    p.template load<NCOMP>(s, pv);
+    pr.slot(st->iter + pr.nslot - 1, lat.V).template load<NCOMP>(s, pv);
 #pragma unroll
     for (int k = 0; k < NCOMP; ++k) {
       const R rr = c[k].re - (res[k].re + s0 * xv[k].re);
@@ -426,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_wilson(Lat lat, SiteMap map, SpinorV
   if (nrhs > 1) batch_block(nrhs, sblk, rhs);
   const Epi e = epi.at(rhs);
   if (!e.active()) return;
-  const SpinorView<P> inr = rhs_view(in, rhs, in_stride);
+  const SpinorView<P> inr = epi_in_view(e, in, rhs, in_stride);
   constexpr int NR = Epi::NRED > 0 ? Epi::NRED : 1;
   double red[NR];
 #pragma unroll
@@ -540,7 +622,7 @@ __global__ __launch_bounds__(ADJ ? kTileAdjThreads
   if (nrhs > 1) batch_block(nrhs, tblk, rhs);
   const Epi e = epi.at(rhs);
   if (!e.active()) return;  // uniform per block (state of one RHS)
-  const SpinorView<P> inr = rhs_view(in, rhs, in_stride);
+  const SpinorView<P> inr = epi_in_view(e, in, rhs, in_stride);
   // tile origin: XCD slab, then x0 fastest, x2, x1
   const int xcd = tblk % kNumXcd;
   const int j = tblk / kNumXcd;
@@ -674,7 +756,7 @@ __global__ __launch_bounds__(kTileMaxThreadsMr, 2) void k_wilson_tile_mr(Lat lat
   TileView<P> view[RB];
 #pragma unroll
   for (int r = 0; r < RB; ++r) {
-    view[r].g = rhs_view(in, min(rhs0 + r, nrhs - 1), in_stride);
+    view[r].g = epi_in_view(e.v[r], in, min(rhs0 + r, nrhs - 1), in_stride);
     view[r].lds = lds + (int64_t)r * NCOMP * tm.nsite;
     view[r].nsite = tm.nsite;
     view[r].self = tid;
@@ -914,6 +996,13 @@ int wilson_launch(EpiKind kind, const OpDesc& d, int prec, bool proj, Field in, 
     case EPI_CGM_K2: return launch_prec<EpiCgmK2>(prec, d, proj, in, a, rc, stream, nrhs);
     case EPI_CGM_K2F: return launch_prec<EpiCgmK2F>(prec, d, proj, in, a, rc, stream, nrhs);
     case EPI_CG_K2X: return launch_prec<EpiCgK2X>(prec, d, proj, in, a, rc, stream, nrhs);
+    case EPI_CG_K1R:
+      if (prec == P64) {
+        set_error("the mixed CG's direction ring is complex64 or 16-bit");
+        return LYM_ERR_UNSUPPORTED;
+      }
+      if (prec == P16) return launch_geom<P16, EpiCgK1R>(d, proj, in, a, rc, stream, nrhs);
+      return launch_geom<P32, EpiCgK1R>(d, proj, in, a, rc, stream, nrhs);
     case EPI_RU1:
     case EPI_RU2:
     case EPI_RU2H:
