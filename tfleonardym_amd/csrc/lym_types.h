@@ -286,6 +286,16 @@ struct SpinorView {
       for (int g = 0; g < G; ++g) v[q * G + g] = mk<R>((float)u.v[2 * g] * sc, (float)u.v[2 * g + 1] * sc);
     }
   }
+  // 16-bit storage: site s's packed integer units and its norm, undecoded
+  template <int NCOMP>
+  __device__ __forceinline__ void load_packed(int64_t s, typename Pack16<NCOMP>::unit (&u)[NCOMP / Pack16<NCOMP>::G],
+                                              float& n) const {
+    using U = typename Pack16<NCOMP>::unit;
+    const uint32_t bytes = (uint32_t)(NCOMP * V * (int64_t)sizeof(E));
+    n = ld<float>(norm, (uint32_t)(V * 4), 0, V, s);
+#pragma unroll
+    for (int q = 0; q < NCOMP / Pack16<NCOMP>::G; ++q) u[q] = ld<U>((const U*)data, bytes, q, V, s);
+  }
   template <int NCOMP>
   __host__ __device__ __forceinline__ void load_raw(int64_t s, cpx<R> (&v)[NCOMP], R& n) const {
     if constexpr (P == P16) {

@@ -533,26 +533,61 @@ __global__ __launch_bounds__(kBlock) void k_wilson(Lat lat, SiteMap map, SpinorV
 // tile read global memory.  Global spinor reads per site drop from 1 + 2D to
 // 1 + sum_{mu<3} 2/a_mu (a direction the tile spans completely costs none).
 // ---------------------------------------------------------------------------
+// 16-bit storage keeps the tile raw in LDS: the packed integer units
+// [NCOMP / G][nsite] (4G bytes each, as in global memory), then the per-site
+// norms [nsite]; 52 B per site at NCOMP = 12 against 96 decoded.
+template <int P, int NCOMP>
+__host__ __device__ constexpr size_t tile_lds_bytes(int nsite) {
+  if constexpr (P == P16)
+    return (size_t)nsite * (NCOMP * 4 + 4);
+  else
+    return (size_t)nsite * NCOMP * sizeof(typename Store<P>::lds_elem);
+}
+
 template <int P>
 struct TileView {
   using R = typename Store<P>::R;
-  using E = typename Store<P>::lds_elem;  // decoded values (float2 for half storage)
+  using E = typename Store<P>::lds_elem;
   SpinorView<P> g;  // the field in global memory (hops leaving the tile)
-  const E* lds;     // [NCOMP][nsite]
+  const E* lds;     // [NCOMP][nsite] decoded (complex128 / complex64), raw for 16-bit
   int nsite, self;  // tile size, this thread's tile index
   int t[4];         // tile coordinates of this thread
   int a[4];         // tile extents
   int str[4];       // tile index strides
   bool full[4];     // the tile spans the whole lattice in mu (wrap inside)
 
+  // 16-bit storage: the integers of tile site idx as floats; returns its norm
+  template <int NCOMP>
+  __device__ __forceinline__ R raw16(int idx, cpx<R> (&v)[NCOMP]) const {
+    using U = typename Pack16<NCOMP>::unit;
+    constexpr int G = Pack16<NCOMP>::G;
+    const U* lu = reinterpret_cast<const U*>(lds);
+#pragma unroll
+    for (int q = 0; q < NCOMP / G; ++q) {
+      const U u = lu[q * nsite + idx];
+#pragma unroll
+      for (int gg = 0; gg < G; ++gg) v[q * G + gg] = mk<R>((R)u.v[2 * gg], (R)u.v[2 * gg + 1]);
+    }
+    return reinterpret_cast<const float*>(lu + (NCOMP / G) * nsite)[idx];
+  }
+  template <int NCOMP>
+  __device__ __forceinline__ void tile_value(int idx, cpx<R> (&v)[NCOMP]) const {
+    if constexpr (P == P16) {
+      const R n = raw16<NCOMP>(idx, v);
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) v[c] = mk<R>(v[c].re * n, v[c].im * n);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NCOMP; ++c) {
+        const E e = lds[c * nsite + idx];
+        v[c] = mk<R>(e.x, e.y);
+      }
+    }
+  }
   // center value: this thread's own site
   template <int NCOMP>
   __device__ __forceinline__ void load(int64_t, cpx<R> (&v)[NCOMP]) const {
-#pragma unroll
-    for (int c = 0; c < NCOMP; ++c) {
-      const E e = lds[c * nsite + self];
-      v[c] = mk<R>(e.x, e.y);
-    }
+    tile_value<NCOMP>(self, v);
   }
   // neighbour in direction mu (dir 0: +mu, 1: -mu); nb is its global site
   template <int NCOMP>
@@ -567,18 +602,13 @@ struct TileView {
       inside = full[mu];
     }
     if (inside) {
-      const int idx = self + (tc - t[mu]) * str[mu];
-#pragma unroll
-      for (int c = 0; c < NCOMP; ++c) {
-        const E e = lds[c * nsite + idx];
-        v[c] = mk<R>(e.x, e.y);
-      }
+      tile_value<NCOMP>(self + (tc - t[mu]) * str[mu], v);
     } else {
       g.template load<NCOMP>(nb, v);
     }
   }
-  // 16-bit storage: a neighbour outside the tile comes as integers and its
-  // norm (value = v * n); inside, the decoded LDS value with n = 1
+  // 16-bit storage: a neighbour comes as integers and its norm (value =
+  // v * n), from the raw LDS tile or from global memory
   static constexpr bool kScaled = SpinorView<P>::kScaled;
   template <int NCOMP>
   __device__ __forceinline__ void load_hop_scaled(int mu, int dir, int64_t nb, cpx<R> (&v)[NCOMP], R& n) const {
@@ -593,12 +623,12 @@ struct TileView {
     }
     if (inside) {
       const int idx = self + (tc - t[mu]) * str[mu];
-#pragma unroll
-      for (int c = 0; c < NCOMP; ++c) {
-        const E e = lds[c * nsite + idx];
-        v[c] = mk<R>(e.x, e.y);
+      if constexpr (P == P16) {
+        n = raw16<NCOMP>(idx, v);
+      } else {
+        tile_value<NCOMP>(idx, v);
+        n = R(1);
       }
-      n = R(1);
     } else {
       g.template load_raw<NCOMP>(nb, v, n);
     }
@@ -644,7 +674,17 @@ __global__ __launch_bounds__(ADJ ? kTileAdjThreads
   x[3] = t3;
   const int64_t s = (((int64_t)x[0] * lat.L[1] + x[1]) * lat.L[2] + x[2]) * tm.T + x[3];
   E* lds = reinterpret_cast<E*>(tile_smem);
-  {
+  if constexpr (P == P16) {
+    using U = typename Pack16<NCOMP>::unit;
+    constexpr int G = Pack16<NCOMP>::G;
+    U u[NCOMP / G];
+    float n;
+    inr.template load_packed<NCOMP>(s, u, n);
+    U* lu = reinterpret_cast<U*>(tile_smem);
+#pragma unroll
+    for (int k = 0; k < NCOMP / G; ++k) lu[k * tm.nsite + tid] = u[k];
+    reinterpret_cast<float*>(lu + (NCOMP / G) * tm.nsite)[tid] = n;
+  } else {
     cpx<R> c[NCOMP];
     inr.template load<NCOMP>(s, c);
 #pragma unroll
@@ -889,7 +929,7 @@ static int launch_one(const OpDesc& d, bool proj, Field in, const EpiArgs& a, Re
   if constexpr (D == 4 && !ADJ) {
     if (proj && wilson_tiled(d, P)) {
       const TileMap& tm = d.tile[P == P64 ? 0 : 1];
-      const size_t lds = (size_t)NCOMP * tm.nsite * sizeof(typename Store<P>::lds_elem);
+      const size_t lds = tile_lds_bytes<P, NCOMP>(tm.nsite);
       const dim3 grid((unsigned)tm.blocks * (unsigned)nrhs), block(tm.nsite);
       // hops in flight: 4 is built for complex64 only (complex128 would
       // spill); more than one needs blocks of at most 512 (complex128: 320)
