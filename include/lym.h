@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define LYM_ABI_VERSION 5
+#define LYM_ABI_VERSION 6
 
 enum {
   LYM_OK = 0,
@@ -219,6 +219,9 @@ typedef struct lym_tile_choice {
   int32_t a0, a1, k, hif;
   int32_t nsite;        /* sites (threads) per block */
   int32_t tuned;        /* 1: from lym_tune_wilson */
+  int32_t walk;         /* order of an XCD's tiles: 0 = x0 fastest, then x2,
+                           x1; 1 = x0, then x1, x2 (tuned with the shape) */
+  int32_t reserved;
   double us;            /* measured time per launch (lym_tune_wilson only) */
 } lym_tile_choice;
 int lym_tune_wilson(const lym_wilson* op, int prec, int reps, const void* in, void* out, lym_tile_choice* best,

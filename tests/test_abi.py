@@ -52,6 +52,7 @@ def test_struct_layout_matches_header():
     from tfleonardym_amd import _lib
     # 4 int32 + 4 int32 dims + 3 double + 4 pointers + link_format + reserved
     assert ctypes.sizeof(_lib.LymWilson) == 4 * 4 + 4 * 4 + 3 * 8 + 4 * 8 + 2 * 4
+    assert ctypes.sizeof(_lib.LymTileChoice) == 8 * 4 + 8
     assert ctypes.sizeof(_lib.LymSolveStats) == 4 * 4 + 8 + 8 + 64 * 4 + 64 * 8 + 64 * 8 + 64 * 4 + 2 * 4
 
 

@@ -92,7 +92,8 @@ def _tile_shapes(geometry, es, ncomp, max_sites=None):
 def test_lds_tile_shapes(prec, geometry, monkeypatch):
     """Every admissible LDS tile shape (forced through LYM_TILE) gives the
     oracle's D, and the same result as the untiled kernel (LYM_NO_TILE=1) to
-    rounding: partial tiles, full-extent wrap, tile edges in every direction."""
+    rounding: partial tiles, full-extent wrap, tile edges in every direction.
+    The other walk order over the tiles gives the identical result."""
     import ctypes
     from tfleonardym_amd import _lib
     print(f"case lds tiles prec={prec} geometry={geometry}", flush=True)
@@ -126,7 +127,11 @@ def test_lds_tile_shapes(prec, geometry, monkeypatch):
             assert rel(got.to(torch.complex128), want) < tol, (a0, a1, k, hif)
             d = (got - plain).abs().max().item() / plain.abs().max().item()
             assert d < (1e-15 if prec == "f64" else 1e-6), (a0, a1, k, hif, d)
+            if hif == 1:
+                walk0 = got
         monkeypatch.delenv("LYM_HIF")
+        monkeypatch.setenv("LYM_TILE", f"{a0},{a1},{k},1")
+        assert torch.equal(run(), walk0), (a0, a1, k, "walk 1")
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32"])

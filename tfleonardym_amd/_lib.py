@@ -15,7 +15,7 @@ PREC_F64, PREC_F32, PREC_F16 = 0, 1, 2
 REP_FUNDAMENTAL, REP_ADJOINT = 0, 1
 MODE_PROJECTED, MODE_SIMPLE = 0, 1
 LINKS_FULL, LINKS_SU = 0, 1
-ABI_VERSION = 5  # LYM_ABI_VERSION of include/lym.h
+ABI_VERSION = 6  # LYM_ABI_VERSION of include/lym.h
 
 OK = 0
 NOT_CONVERGED = 1
@@ -43,7 +43,8 @@ class LymWilson(ctypes.Structure):
 
 
 class LymTileChoice(ctypes.Structure):
-    """lym_tile_choice: LDS tile (a0, a1, k; a0 = 0: untiled) and hops in flight."""
+    """lym_tile_choice: LDS tile (a0, a1, k; a0 = 0: untiled), hops in flight
+    and the order in which an XCD's blocks walk the tiles."""
     _fields_ = [
         ("a0", ctypes.c_int32),
         ("a1", ctypes.c_int32),
@@ -51,6 +52,8 @@ class LymTileChoice(ctypes.Structure):
         ("hif", ctypes.c_int32),
         ("nsite", ctypes.c_int32),
         ("tuned", ctypes.c_int32),
+        ("walk", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("us", ctypes.c_double),
     ]
 

@@ -469,6 +469,7 @@ struct TileMap {
   int nsite;       // a0 * a1 * k * T (block size)
   int blocks;      // tiles in the lattice
   int hif;         // hops in flight per wave: 1, 2 or 4 (0 = 1)
+  int walk;        // order of an XCD's tiles: 0 = x0 fastest, then x2, x1; 1 = x0, then x1, x2
 };
 
 template <int D>

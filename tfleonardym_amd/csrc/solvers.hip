@@ -1851,9 +1851,13 @@ static int batched_tile(const OpDesc& d, const Parts& P, int pl, int nrhs, Field
   for (const TileMap& t : tile_candidates(d, es, tile_max_threads(0, pl), kTileMaxLds)) {
     for (int hif : {1, 2}) {
       if (hif > 1 && t.nsite > tile_hif_threads(pl)) continue;
-      TileMap x = t;
-      x.hif = hif;
-      cands.push_back(x);
+      for (int walk : {0, 1}) {
+        if (walk && (t.n1 == 1 || t.n2 == 1)) continue;  // the same order
+        TileMap x = t;
+        x.hif = hif;
+        x.walk = walk;
+        cands.push_back(x);
+      }
     }
   }
   if (cands.empty()) return LYM_OK;

@@ -653,13 +653,13 @@ __global__ __launch_bounds__(ADJ ? kTileAdjThreads
   const Epi e = epi.at(rhs);
   if (!e.active()) return;  // uniform per block (state of one RHS)
   const SpinorView<P> inr = epi_in_view(e, in, rhs, in_stride);
-  // tile origin: XCD slab, then x0 fastest, x2, x1
+  // tile origin: XCD slab, then x0 fastest, x2, x1 (walk 1: x0, x1, x2)
   const int xcd = tblk % kNumXcd;
   const int j = tblk / kNumXcd;
   const int i0 = j % tm.n0;
   const int r = j / tm.n0;
-  const int i2 = r % tm.n2;
-  const int i1 = r / tm.n2;
+  const int i2 = tm.walk ? r / tm.n1 : r % tm.n2;
+  const int i1 = tm.walk ? r % tm.n1 : r / tm.n2;
   const int tid = (int)threadIdx.x;
   int q = tid / tm.T;
   const int t3 = tid - q * tm.T;
@@ -777,8 +777,8 @@ __global__ __launch_bounds__(kTileMaxThreadsMr, 2) void k_wilson_tile_mr(Lat lat
   const int j = tblk / kNumXcd;
   const int i0 = j % tm.n0;
   const int rr = j / tm.n0;
-  const int i2 = rr % tm.n2;
-  const int i1 = rr / tm.n2;
+  const int i2 = tm.walk ? rr / tm.n1 : rr % tm.n2;
+  const int i1 = tm.walk ? rr % tm.n1 : rr / tm.n2;
   const int tid = (int)threadIdx.x;
   int q = tid / tm.T;
   const int t3 = tid - q * tm.T;
