@@ -454,6 +454,7 @@ struct SiteMap {
   int k;     // lines per block
   int slab;  // x0 values per XCD
   int ng;    // L_2 / k
+  int walk;  // order of an XCD's blocks after x0: 0 = x2 groups, then x1; 1 = x1, then x2 groups
 };
 
 // LDS-tiled Wilson kernel (4D): a block stages the input spinor of a tile of

@@ -306,9 +306,9 @@ __host__ __device__ __forceinline__ bool site_of(const Lat& lat, const SiteMap& 
       const int j = sblk / kNumXcd;
       const int x0l = j % m.slab;
       const int r = j / m.slab;
-      const int g = r % m.ng;
+      const int g = m.walk ? r / lat.L[1] : r % m.ng;
       x[0] = xcd * m.slab + x0l;
-      x[1] = r / m.ng;
+      x[1] = m.walk ? r % lat.L[1] : r / m.ng;
       x[2] = g * m.k + tid / m.T;
       x[3] = tid % m.T;
       s = (((int64_t)x[0] * lat.L[1] + x[1]) * lat.L[2] + x[2]) * m.T + x[3];
